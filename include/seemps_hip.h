@@ -1,0 +1,193 @@
+/*
+ * seemps_hip.h -- C ABI of libseemps_hip.so, the gfx950 (MI355X) kernels of the
+ * SeeMPS tensor-train hot path.
+ *
+ * Conventions
+ *   - Every pointer to tensor data is a DEVICE pointer (HIP global memory).
+ *   - complex128 = interleaved (re, im) doubles; tensors are row-major (C order),
+ *     with the layouts of the reference: MPS A[a, i, b], MPO W[a, i_out, j_in, b],
+ *     MPS environments rho[bra, ket], MPO environments L/R[bra, mpo, ket].
+ *   - `stream` is a hipStream_t (NULL = default stream).  All work is enqueued on
+ *     it; entry points that must return a size to the host (the kept bond
+ *     dimension D, eigensolver statistics) synchronise that stream once.
+ *   - Scratch memory is caller-provided (`work`, `work_bytes`); the library never
+ *     allocates persistent device memory.  `*_workspace()` gives the bytes needed.
+ *   - Return value: SEEMPS_OK or an error code; seemps_last_error() describes it.
+ *     The Python layer maps SEEMPS_ERR_INVALID -> ValueError,
+ *     SEEMPS_ERR_NOCONVERGE -> scipy.linalg.LinAlgError, SEEMPS_ERR_RUNTIME ->
+ *     RuntimeError, as the reference does (src/seemps/cython/svd.pxi:78-83,
+ *     contractions.pxi:37, environments.pxi:17,48).
+ *
+ * Citations are to the reference repository (juanjosegarciaripoll/seemps2).
+ */
+#ifndef SEEMPS_HIP_H
+#define SEEMPS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEEMPS_OK 0
+#define SEEMPS_ERR_INVALID 1
+#define SEEMPS_ERR_NOCONVERGE 2
+#define SEEMPS_ERR_RUNTIME 3
+#define SEEMPS_ERR_WORKSPACE 4
+
+/* Operand flags.  0/1/2 are GemmOrder.NORMAL/TRANSPOSE/ADJOINT of the reference
+ * (src/seemps/cython/gemm.pxi:8-11, _python_core.py:48-54); 3 = conjugate. */
+#define SEEMPS_OP_N 0
+#define SEEMPS_OP_T 1
+#define SEEMPS_OP_H 2
+#define SEEMPS_OP_C 3
+
+/* Truncation method codes: Truncation.* of src/seemps/cython/truncation.pxi:1-8. */
+#define SEEMPS_TRUNC_NONE 0
+#define SEEMPS_TRUNC_REL_SV 1
+#define SEEMPS_TRUNC_REL_NORM2 2
+#define SEEMPS_TRUNC_ABS_SV 3
+
+/* Message for the last error raised on the calling host thread. */
+const char* seemps_last_error(void);
+/* ABI version of this header (incremented on incompatible changes). */
+int seemps_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Contractions
+ * ------------------------------------------------------------------------- */
+
+/* C[b] = alpha * op(A[b]) @ op(B[b]) + beta * C[b], complex128, b < batch.
+ * op(A) is M x K, op(B) is K x N; lda/ldb/ldc are row strides (elements) of the
+ * stored matrices, stride* the batch strides (elements).  beta == 0 never reads C.
+ * Replaces: _gemm (src/seemps/cython/gemm.pxi:16-122), _contract_last_and_first
+ * (contractions.pxi:6-38), the two GEMMs of _update_left/right_environment
+ * (environments.pxi:10-62) and np.matmul in AntilinearForm.tensor2site
+ * (state/antilinear.py:96). */
+int seemps_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
+                 double alpha_re, double alpha_im,
+                 const void* A, int64_t lda, int64_t strideA,
+                 const void* B, int64_t ldb, int64_t strideB,
+                 double beta_re, double beta_im,
+                 void* C, int64_t ldc, int64_t strideC,
+                 int64_t batch, void* stream);
+
+/* Local-operator contraction, complex128:
+ *   out[p, j1, q, j2] = sum_i in[p, i, q] * Mop[i, j1, j2]   (+ out if accumulate)
+ * with in[P, I, Q], Mop[I, J1, J2], out[P, J1, Q, J2] contiguous, 64*I + I*J1*J2 <= 3072.  Replaces the small-index steps of _mpo_multiply_tensor
+ * (src/seemps/operators/mpo.py:26-45, "cjd,aijb->caidb"), of the MPO environment
+ * updates (state/environments.py:19-42), of the two-site matvec
+ * (operators/quadratic.py:204-214) and of _contract_nrjl_ijk_klm
+ * (cython/contractions.pxi:102-132). */
+int seemps_zlocal(int64_t P, int64_t I, int64_t Q, int64_t J1, int64_t J2,
+                  const void* in, const void* Mop, void* out, int accumulate,
+                  void* stream);
+
+/* y[k] = a * op(x[k]) + b * y[k] for k < n (op = N or C); b == 0 never reads y. */
+int seemps_zaxpby(int64_t n, double a_re, double a_im, const void* x, int conj_x,
+                  double b_re, double b_im, void* y, void* stream);
+
+/* dst[r, c] = sr[r] * src[r, c] * sc[c] for r < rows, c < cols, with optional
+ * real scalings sr / sc (NULL = 1) and row strides ld_src / ld_dst (elements).
+ * Used for the s*V / U*s products of schmidt.pxi:40,81,145,171. */
+int seemps_zscale2d(int64_t rows, int64_t cols, const double* sr, const double* sc,
+                    const void* src, int64_t ld_src, void* dst, int64_t ld_dst,
+                    void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Reductions (deterministic two-pass wavefront reductions)
+ * ------------------------------------------------------------------------- */
+
+int64_t seemps_reduce_workspace(int64_t n);
+/* result[0] = sum_k conj(x[k]) * y[k] (device complex128).  np.vdot as used by
+ * simplify (state/simplification.py:125-126) and _join_environments
+ * (cython/environments.pxi:79-85). */
+int seemps_zdotc(int64_t n, const void* x, const void* y, void* result,
+                 void* work, int64_t work_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Truncation and SVD
+ * ------------------------------------------------------------------------- */
+
+/* Truncation rule on a descending FP64 spectrum s[0..N): writes the kept size
+ * D_out[0] (device int64) and the squared error err_out[0] (device double).
+ * Replaces destructively_truncate_vector / Strategy._truncate
+ * (src/seemps/cython/truncation.pxi:222-334), same loops and summation order. */
+int seemps_truncate(int64_t N, const double* s, int method, double tolerance,
+                    int64_t max_bond, int64_t* D_out, double* err_out, void* stream);
+
+/* Economy SVD A = U diag(S) VH of an m x n complex128 matrix (row stride lda),
+ * S descending, U m x r (row stride r), VH r x n (row stride n), r = min(m, n).
+ * One-sided block Jacobi in FP64 on one device (cooperative persistent kernel).
+ * info (host, may be NULL) receives {sweeps, converged}.  Returns
+ * SEEMPS_ERR_NOCONVERGE if the sweep limit is reached.
+ * Replaces _destructive_svd (src/seemps/cython/svd.pxi:17-83). */
+int64_t seemps_svd_workspace(int64_t m, int64_t n);
+int seemps_zgesvd(int64_t m, int64_t n, const void* A, int64_t lda, void* U,
+                  double* S, void* VH, void* work, int64_t work_bytes, int32_t* info,
+                  void* stream);
+
+/* SVD + truncation + split of an m x n matrix (the two-site tensor reshaped as
+ * (a*d1, d2*b)).  mode 0 = left-orthogonal split: outA = U[:, :D] (m x D),
+ * outB = diag(s[:D]) VH[:D] (D x n) -- _left_orth_2site (schmidt.pxi:123-147),
+ * __update_in_canonical_form_right (schmidt.pxi:18-42).
+ * mode 1 = right-orthogonal split: outA = U[:, :D] diag(s[:D]), outB = VH[:D]
+ * -- _right_orth_2site (schmidt.pxi:150-174), __update_in_canonical_form_left
+ * (schmidt.pxi:59-82).  outA must hold m*r elements, outB r*n, s_out r doubles
+ * (may be NULL).  D_host / err_host receive D and the squared truncation error
+ * (the stream is synchronised once). */
+int64_t seemps_svd_split_workspace(int64_t m, int64_t n);
+int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, int method,
+                     double tolerance, int64_t max_bond, void* outA, void* outB,
+                     double* s_out, int64_t* D_host, double* err_host, void* work,
+                     int64_t work_bytes, int32_t* info, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * DMRG two-site effective operator and eigensolver
+ * ------------------------------------------------------------------------- */
+
+/* out[a, ko, lo, g] = sum L[a, c, b] H[c, ko, k, lo, l, d] v[b, k, l, f] R[g, d, f]
+ * where the MPO pair enters as Hm[(c, k, l), (ko, lo, d)] (w*d1*d2 x d1*d2*w2).
+ * Shapes: L (a, c, b), v (b, d1, d2, f), R (g, w2, f).  Work: two intermediates.
+ * Replaces QuadraticForm.two_site_operator._matvec (operators/quadratic.py:187-214). */
+int64_t seemps_two_site_matvec_workspace(int64_t a, int64_t c, int64_t b, int64_t d1,
+                                         int64_t d2, int64_t w2, int64_t f);
+int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2,
+                           int64_t w2, int64_t f, int64_t g, const void* L,
+                           const void* Hm, const void* R, const void* v, void* out,
+                           void* work, int64_t work_bytes, void* stream);
+
+/* Hm for the MPO pair (O1, O2): Hm[(c,k,l),(ko,lo,d)] = sum_e O1[c,ko,k,e] O2[e,lo,l,d]
+ * (the precomputed H12 reshape of quadratic.py:103,198-201).  O1 is c x d1 x d1 x e,
+ * O2 is e x d2 x d2 x w2. */
+int seemps_two_site_operator(int64_t c, int64_t d1, int64_t e, int64_t d2, int64_t w2,
+                             const void* O1, const void* O2, void* Hm, void* stream);
+
+/* out = in.permute(perm) for a contiguous complex128 tensor of ndim <= 8 dims
+ * (out axis k is input axis perm[k]); dims/perm are host arrays. */
+int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, const void* in,
+                    void* out, void* stream);
+
+/* Lowest eigenpair of the Hermitian two-site operator above for bra = ket
+ * (a = b = chiL, g = f = chiR), Lanczos with full re-orthogonalisation, Krylov dimension ncv, explicit
+ * restarts from the Ritz vector, started from v0; stops when the Ritz residual
+ * |beta_m y_m| <= tol * max(eps^(2/3), |theta|) (ARPACK's test; tol = 0 means
+ * machine epsilon).  The Krylov recurrence, the tridiagonal eigensolve (FP64
+ * bisection + inverse iteration) and the Ritz vector stay on the device; the
+ * stream is synchronised once per restart.  evec receives the unit-norm
+ * eigenvector; stats (host, may be NULL) = {eigenvalue, residual, matvecs,
+ * restarts}.  Replaces _diagonalize_two_site (optimization/dmrg.py:15-23), i.e.
+ * scipy eigsh -> ARPACK (scipy/sparse/linalg/_eigen/arpack/arpack.py:1565-1583). */
+int64_t seemps_eigensolve_workspace(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
+                                    int64_t w2, int64_t chiR, int64_t ncv);
+int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t w2,
+                               int64_t chiR, const void* L, const void* Hm, const void* R,
+                               const void* v0, void* evec, int64_t ncv, double tol,
+                               int64_t maxiter, double* stats, void* work,
+                               int64_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEEMPS_HIP_H */

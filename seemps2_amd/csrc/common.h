@@ -1,0 +1,92 @@
+// Shared definitions for the gfx950 kernels of the SeeMPS hot path.
+//
+// Storage conventions (all row-major, as NumPy / torch C-order):
+//   complex128 values are interleaved (re, im) pairs, accessed as double2.
+//   MPS tensor A[a, i, b], MPO tensor W[a, i_out, j_in, b],
+//   MPO environments L[bra, mpo, ket], R[bra, mpo, ket].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/seemps_hip.h"
+
+namespace seemps {
+
+// Last error message of the calling host thread (returned by seemps_last_error).
+void set_error(const char* fmt, ...);
+
+#define SEEMPS_CHECK_ARG(cond, ...)                          \
+  do {                                                       \
+    if (!(cond)) {                                           \
+      ::seemps::set_error(__VA_ARGS__);                      \
+      return SEEMPS_ERR_INVALID;                             \
+    }                                                        \
+  } while (0)
+
+#define SEEMPS_HIP(call)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      ::seemps::set_error("%s failed: %s", #call, hipGetErrorString(e_));  \
+      return SEEMPS_ERR_RUNTIME;                                           \
+    }                                                                      \
+  } while (0)
+
+// Check the most recent launch.
+int check_launch(const char* what);
+
+struct cplx {
+  double re, im;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+  // a * b + c
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+__device__ __forceinline__ double2 cfmac(double2 a, double2 b, double2 c) {
+  // conj(a) * b + c
+  return make_double2(fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y)));
+}
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Device properties cached per device (CU count, cooperative limits).
+struct DeviceInfo {
+  int device;
+  int num_cu;
+};
+int device_info(DeviceInfo* info);
+
+// Internal enqueue functions shared between the translation units.
+int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
+          int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB, double2 beta, void* C,
+          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream);
+int zaxpby_impl(int64_t n, double2 a, const double2* x, int conj_x, double2 b, double2* y,
+                hipStream_t stream);
+int zlocal_impl(int64_t P, int64_t I, int64_t Q, int64_t J1, int64_t J2, const double2* in,
+                const double2* Mop, double2* out, int accumulate, hipStream_t stream);
+int zscale2d_impl(int64_t rows, int64_t cols, const double* sr, const double* sc,
+                  const double2* src, int64_t ld_src, double2* dst, int64_t ld_dst,
+                  hipStream_t stream);
+// Number of partial sums used by the two-pass reductions for length n.
+int64_t reduce_blocks(int64_t n);
+// result[j] = sum_k conj(V[j*ldv + k]) * w[k], j < nv (nv <= 64); work holds
+// reduce_blocks(n) * nv partials.
+int zmdotc_impl(int64_t n, int64_t nv, const double2* V, int64_t ldv, const double2* w,
+                double2* result, double2* work, hipStream_t stream);
+
+}  // namespace seemps

@@ -1,0 +1,499 @@
+// DMRG two-site effective Hamiltonian: operator build, matvec and a Lanczos
+// eigensolver whose Krylov recurrence, tridiagonal eigensolve and Ritz vector
+// stay on the device (one host synchronisation per restart cycle).
+//
+// Replaces QuadraticForm.two_site_operator (src/seemps/operators/quadratic.py:
+// 187-237) and the scipy eigsh / ARPACK call of _diagonalize_two_site
+// (src/seemps/optimization/dmrg.py:15-23).
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace seemps {
+namespace {
+
+// Hm[(c,k,l), (ko,lo,d)] = sum_e O1[c, ko, k, e] O2[e, lo, l, d]
+__global__ void two_site_op_kernel(int c_, int d1, int e_, int d2, int w2,
+                                   const double2* __restrict__ O1, const double2* __restrict__ O2,
+                                   double2* __restrict__ Hm) {
+  const int rows = c_ * d1 * d2, cols = d1 * d2 * w2;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int row = idx / cols, col = idx % cols;
+  const int c = row / (d1 * d2), k = (row / d2) % d1, l = row % d2;
+  const int ko = col / (d2 * w2), lo = (col / w2) % d2, d = col % w2;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int e = 0; e < e_; ++e) {
+    const double2 a = O1[((c * d1 + ko) * d1 + k) * e_ + e];
+    const double2 b = O2[((e * d2 + lo) * d2 + l) * w2 + d];
+    acc = cfma(a, b, acc);
+  }
+  Hm[idx] = acc;
+}
+
+struct PermDesc {
+  int ndim;
+  int64_t out_dims[8];
+  int64_t in_strides_perm[8];  // stride in the input of output axis k
+};
+
+__global__ void permute_kernel(int64_t total, PermDesc p, const double2* __restrict__ in,
+                               double2* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rem = e, src = 0;
+    for (int k = p.ndim - 1; k >= 0; --k) {
+      const int64_t i = rem % p.out_dims[k];
+      rem /= p.out_dims[k];
+      src += i * p.in_strides_perm[k];
+    }
+    out[e] = in[src];
+  }
+}
+
+// w[i] -= sum_{j<k} V[j*ldv + i] * h[j]
+__global__ void maxpy_kernel(int64_t n, int k, const double2* __restrict__ V, int64_t ldv,
+                             const double2* __restrict__ h, double2* __restrict__ w) {
+  __shared__ double2 hs[64];
+  if (threadIdx.x < k) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = w[i];
+    for (int j = 0; j < k; ++j) {
+      const double2 t = cmul(V[j * ldv + i], hs[j]);
+      acc.x -= t.x;
+      acc.y -= t.y;
+    }
+    w[i] = acc;
+  }
+}
+
+// x[i] = sum_{j<k} V[j*ldv + i] * y[j]  (y real)
+__global__ void combine_kernel(int64_t n, int k, const double2* __restrict__ V, int64_t ldv,
+                               const double* __restrict__ y, double2* __restrict__ x) {
+  __shared__ double ys[64];
+  if (threadIdx.x < k) ys[threadIdx.x] = y[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < k; ++j) {
+      const double2 v = V[j * ldv + i];
+      acc.x += v.x * ys[j];
+      acc.y += v.y * ys[j];
+    }
+    x[i] = acc;
+  }
+}
+
+// Scalars of one Lanczos cycle kept on the device.
+struct LzScalars {
+  double alpha[64];
+  double beta[64];
+  double y[64];
+  double2 h[64];
+  double2 h2[64];
+  double2 nrm;      // <w, w>
+  double theta;
+  double resid;
+  double scale;     // running max |alpha| + |beta| (breakdown reference)
+  int kd;           // effective Krylov dimension of the cycle
+  int broken;       // breakdown seen in this cycle
+};
+
+// After the two orthogonalisation passes of step j: record alpha_j, beta_j.
+__global__ void lz_record(LzScalars* s, int j) {
+  if (threadIdx.x != 0) return;
+  const double a = s->h[j].x + s->h2[j].x;
+  s->alpha[j] = a;
+  const double b = sqrt(fmax(s->nrm.x, 0.0));
+  const double prev = (j > 0) ? s->beta[j - 1] : 0.0;
+  const double ref = fmax(s->scale, fabs(a) + prev);
+  s->scale = ref;
+  if (s->broken || b <= 1e-12 * ref) {
+    if (!s->broken) {
+      s->broken = 1;
+      s->kd = j + 1;
+    }
+    s->beta[j] = 0.0;
+  } else {
+    s->beta[j] = b;
+  }
+}
+
+// v = w / beta_j (zero vector after a breakdown).
+__global__ void lz_normalize(int64_t n, const LzScalars* __restrict__ s, int j,
+                             const double2* __restrict__ w, double2* __restrict__ v) {
+  const double b = s->beta[j];
+  const double inv = (b > 0.0) ? 1.0 / b : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = w[i];
+    v[i] = make_double2(x.x * inv, x.y * inv);
+  }
+}
+
+// Scale v in place by 1/sqrt(<v,v>) read from s->nrm.
+__global__ void lz_unit(int64_t n, const LzScalars* __restrict__ s, double2* __restrict__ v) {
+  const double nn = sqrt(fmax(s->nrm.x, 0.0));
+  const double inv = (nn > 0.0) ? 1.0 / nn : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = v[i];
+    v[i] = make_double2(x.x * inv, x.y * inv);
+  }
+}
+
+__global__ void lz_reset(LzScalars* s) {
+  if (threadIdx.x == 0) {
+    s->broken = 0;
+    s->kd = 0;
+    s->scale = 0.0;
+  }
+}
+
+// Sturm count: number of eigenvalues of the leading kd x kd block below x.
+__device__ int sturm(const double* a, const double* b, int kd, double x) {
+  int cnt = 0;
+  double d = a[0] - x;
+  if (d < 0.0) ++cnt;
+  for (int i = 1; i < kd; ++i) {
+    const double dd = (d == 0.0) ? 1e-300 : d;
+    d = a[i] - x - b[i - 1] * b[i - 1] / dd;
+    if (d < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+// Lowest eigenpair of the symmetric tridiagonal T = tridiag(beta, alpha, beta)
+// of dimension kd (= m, or the breakdown size): 64-way multisection on Sturm
+// counts for theta, then inverse iteration (tridiagonal LU with partial
+// pivoting, LAPACK dgttrf/dgttrs steps) for y.  One wavefront.
+__global__ __launch_bounds__(64) void lz_tridiag(LzScalars* s, int m) {
+  __shared__ double lo_s, hi_s;
+  const int lane = threadIdx.x;
+  const int kd = s->broken ? s->kd : m;
+  const double* a = s->alpha;
+  const double* b = s->beta;
+  if (lane == 0) {
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < kd; ++i) {
+      const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < kd - 1 ? fabs(b[i]) : 0.0);
+      lo = fmin(lo, a[i] - r);
+      hi = fmax(hi, a[i] + r);
+    }
+    const double pad = 1e-12 * fmax(fabs(lo), fabs(hi)) + 1e-300;
+    lo_s = lo - pad;
+    hi_s = hi + pad;
+  }
+  __syncthreads();
+  for (int round = 0; round < 12; ++round) {
+    const double lo = lo_s, hi = hi_s;
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const int c = sturm(a, b, kd, x);
+    // smallest lane whose point has at least one eigenvalue below it
+    const unsigned long long mask = __ballot(c >= 1);
+    __syncthreads();
+    if (lane == 0) {
+      if (mask) {
+        const int l = __ffsll((long long)mask) - 1;
+        const double nhi = lo + (hi - lo) * (double)(l + 1) / 65.0;
+        const double nlo = (l == 0) ? lo : lo + (hi - lo) * (double)l / 65.0;
+        lo_s = nlo;
+        hi_s = nhi;
+      } else {
+        lo_s = lo + (hi - lo) * 64.0 / 65.0;
+      }
+    }
+    __syncthreads();
+  }
+  if (lane != 0) return;
+  const double theta = 0.5 * (lo_s + hi_s);
+  // Inverse iteration on (T - theta I).
+  double d[64], dl[64], du[64], du2[64], y[64];
+  int piv[64];
+  double tnorm = 0.0;
+  for (int i = 0; i < kd; ++i) tnorm = fmax(tnorm, fabs(a[i]) + (i < kd - 1 ? fabs(b[i]) : 0.0));
+  const double tiny = 2.220446049250313e-16 * fmax(tnorm, 1e-300);
+  for (int i = 0; i < kd; ++i) {
+    d[i] = a[i] - theta;
+    du[i] = (i < kd - 1) ? b[i] : 0.0;
+    dl[i] = (i < kd - 1) ? b[i] : 0.0;
+    du2[i] = 0.0;
+    piv[i] = i;
+  }
+  for (int i = 0; i < kd - 1; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (d[i] == 0.0) d[i] = tiny;
+      const double fact = dl[i] / d[i];
+      dl[i] = fact;
+      d[i + 1] -= fact * du[i];
+      piv[i] = i;
+    } else {
+      const double fact = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = fact;
+      const double temp = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = temp - fact * d[i + 1];
+      if (i < kd - 2) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -fact * du[i + 1];
+      }
+      piv[i] = i + 1;
+    }
+  }
+  for (int i = 0; i < kd; ++i)
+    if (d[i] == 0.0) d[i] = tiny;
+  for (int i = 0; i < kd; ++i) y[i] = 1.0;
+  for (int it = 0; it < 4; ++it) {
+    for (int i = 0; i < kd - 1; ++i) {
+      if (piv[i] == i) {
+        y[i + 1] -= dl[i] * y[i];
+      } else {
+        const double temp = y[i];
+        y[i] = y[i + 1];
+        y[i + 1] = temp - dl[i] * y[i];
+      }
+    }
+    y[kd - 1] /= d[kd - 1];
+    if (kd > 1) y[kd - 2] = (y[kd - 2] - du[kd - 2] * y[kd - 1]) / d[kd - 2];
+    for (int i = kd - 3; i >= 0; --i) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / d[i];
+    double nn = 0.0;
+    for (int i = 0; i < kd; ++i) nn += y[i] * y[i];
+    nn = sqrt(nn);
+    for (int i = 0; i < kd; ++i) y[i] /= nn;
+  }
+  // Fix the sign: largest component positive.
+  int imax = 0;
+  for (int i = 1; i < kd; ++i)
+    if (fabs(y[i]) > fabs(y[imax])) imax = i;
+  const double sg = (y[imax] < 0.0) ? -1.0 : 1.0;
+  for (int i = 0; i < 64; ++i) s->y[i] = (i < kd) ? sg * y[i] : 0.0;
+  s->theta = theta;
+  s->resid = (kd == m && !s->broken) ? fabs(b[m - 1] * y[kd - 1]) : 0.0;
+}
+
+struct EigWork {
+  double2* V;      // (m + 1) x n
+  double2* x1;     // a*c*d1*d2*f
+  double2* x2;     // a*d1*d2*w2*f
+  double2* part;   // reduction partials (64 x nblk)
+  LzScalars* sc;
+};
+
+inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+int64_t eig_bytes_full(int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t w2, int64_t chiR,
+                       int64_t m) {
+  const int64_t n = chiL * d1 * d2 * chiR;
+  int64_t b = 0;
+  b += al256((m + 1) * n * 16);
+  b += al256(chiL * c * d1 * d2 * chiR * 16);
+  b += al256(chiL * d1 * d2 * w2 * chiR * 16);
+  b += al256(64 * reduce_blocks(n) * 16);
+  b += al256(sizeof(LzScalars));
+  return b;
+}
+
+EigWork eig_carve(void* work, int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t w2,
+                  int64_t chiR, int64_t m) {
+  const int64_t n = chiL * d1 * d2 * chiR;
+  char* p = (char*)work;
+  EigWork w;
+  w.V = (double2*)p;
+  p += al256((m + 1) * n * 16);
+  w.x1 = (double2*)p;
+  p += al256(chiL * c * d1 * d2 * chiR * 16);
+  w.x2 = (double2*)p;
+  p += al256(chiL * d1 * d2 * w2 * chiR * 16);
+  w.part = (double2*)p;
+  p += al256(64 * reduce_blocks(n) * 16);
+  w.sc = (LzScalars*)p;
+  return w;
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t b = ceil_div(n, 256);
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+// out[a, ko, lo, g] = sum L[a,c,b] Hm[(c,k,l),(ko,lo,d)] v[b,k,l,f] R[g,d,f]
+int two_site_matvec_impl(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2, int64_t w2,
+                         int64_t f, int64_t g, const double2* L, const double2* Hm,
+                         const double2* R, const double2* v, double2* out, double2* x1,
+                         double2* x2, hipStream_t stream) {
+  int st;
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // x1[(a,c), (k,l,f)] = L[(a,c), b] v[b, (k,l,f)]
+  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, a * c, d1 * d2 * f, b, one, L, b, 0, v, d1 * d2 * f,
+                  0, zero, x1, d1 * d2 * f, 0, 1, stream)))
+    return st;
+  // x2[a, (ko,lo,d), f] = sum_{(c,k,l)} x1[a, (c,k,l), f] Hm[(c,k,l), (ko,lo,d)]
+  if ((st = zlocal_impl(a, c * d1 * d2, f, d1 * d2 * w2, 1, x1, Hm, x2, 0, stream))) return st;
+  // out[(a,ko,lo), g] = x2[(a,ko,lo), (d,f)] R[g, (d,f)]^T
+  return zgemm(SEEMPS_OP_N, SEEMPS_OP_T, a * d1 * d2, g, w2 * f, one, x2, w2 * f, 0, R, w2 * f, 0,
+               zero, out, g, 0, 1, stream);
+}
+
+}  // namespace seemps
+
+using namespace seemps;
+
+extern "C" int seemps_two_site_operator(int64_t c, int64_t d1, int64_t e, int64_t d2, int64_t w2,
+                                        const void* O1, const void* O2, void* Hm, void* stream) {
+  SEEMPS_CHECK_ARG(c >= 1 && d1 >= 1 && e >= 1 && d2 >= 1 && w2 >= 1,
+                   "two_site_operator: bad dimensions");
+  const int64_t total = c * d1 * d2 * d1 * d2 * w2;
+  hipLaunchKernelGGL(two_site_op_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0,
+                     (hipStream_t)stream, (int)c, (int)d1, (int)e, (int)d2, (int)w2,
+                     (const double2*)O1, (const double2*)O2, (double2*)Hm);
+  return check_launch("two_site_operator");
+}
+
+extern "C" int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, const void* in,
+                               void* out, void* stream) {
+  SEEMPS_CHECK_ARG(ndim >= 1 && ndim <= 8, "permute: 1 <= ndim <= 8");
+  PermDesc p;
+  p.ndim = ndim;
+  int64_t strides[8];
+  int64_t acc = 1;
+  for (int k = ndim - 1; k >= 0; --k) {
+    strides[k] = acc;
+    acc *= dims[k];
+  }
+  int seen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < ndim; ++k) {
+    SEEMPS_CHECK_ARG(perm[k] >= 0 && perm[k] < ndim && !seen[perm[k]], "permute: invalid perm");
+    seen[perm[k]] = 1;
+    p.out_dims[k] = dims[perm[k]];
+    p.in_strides_perm[k] = strides[perm[k]];
+  }
+  const int64_t total = acc;
+  if (total == 0) return SEEMPS_OK;
+  hipLaunchKernelGGL(permute_kernel, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
+                     total, p, (const double2*)in, (double2*)out);
+  return check_launch("permute");
+}
+
+extern "C" int64_t seemps_two_site_matvec_workspace(int64_t a, int64_t c, int64_t b, int64_t d1,
+                                                    int64_t d2, int64_t w2, int64_t f) {
+  return al256(a * c * d1 * d2 * f * 16) + al256(a * d1 * d2 * w2 * f * 16);
+}
+
+extern "C" int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2,
+                                      int64_t w2, int64_t f, int64_t g, const void* L,
+                                      const void* Hm, const void* R, const void* v, void* out,
+                                      void* work, int64_t work_bytes, void* stream) {
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_two_site_matvec_workspace(a, c, b, d1, d2, w2, f),
+                   "two_site_matvec: workspace too small");
+  SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "two_site_matvec: w*d1*d2 > 64 not supported");
+  double2* x1 = (double2*)work;
+  double2* x2 = (double2*)((char*)work + al256(a * c * d1 * d2 * f * 16));
+  return two_site_matvec_impl(a, c, b, d1, d2, w2, f, g, (const double2*)L, (const double2*)Hm,
+                              (const double2*)R, (const double2*)v, (double2*)out, x1, x2,
+                              (hipStream_t)stream);
+}
+
+extern "C" int64_t seemps_eigensolve_workspace(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
+                                               int64_t w2, int64_t chiR, int64_t ncv) {
+  const int64_t n = chiL * d1 * d2 * chiR;
+  const int64_t m = ncv < n ? ncv : n;
+  return eig_bytes_full(chiL, c, d1, d2, w2, chiR, m);
+}
+
+extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
+                                          int64_t w2, int64_t chiR, const void* L, const void* Hm,
+                                          const void* R, const void* v0, void* evec, int64_t ncv,
+                                          double tol, int64_t maxiter, double* stats, void* work,
+                                          int64_t work_bytes, void* stream) {
+  hipStream_t st_ = (hipStream_t)stream;
+  SEEMPS_CHECK_ARG(chiL >= 1 && chiR >= 1 && c >= 1 && w2 >= 1 && d1 >= 1 && d2 >= 1,
+                   "eigensolve: bad dimensions");
+  SEEMPS_CHECK_ARG(ncv >= 2 && ncv <= 63, "eigensolve: ncv must be in [2, 63]");
+  SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "eigensolve: w*d1*d2 > 64 not supported");
+  SEEMPS_CHECK_ARG(maxiter >= 1, "eigensolve: maxiter must be positive");
+  const int64_t n = chiL * d1 * d2 * chiR;
+  const int64_t m = ncv < n ? ncv : n;
+  SEEMPS_CHECK_ARG(work_bytes >= eig_bytes_full(chiL, c, d1, d2, w2, chiR, m),
+                   "eigensolve: workspace too small");
+  if (tol <= 0.0) tol = 2.220446049250313e-16;
+  const double eps23 = pow(2.220446049250313e-16, 2.0 / 3.0);
+  EigWork w = eig_carve(work, chiL, c, d1, d2, w2, chiR, m);
+  const double2* L_ = (const double2*)L;
+  const double2* H_ = (const double2*)Hm;
+  const double2* R_ = (const double2*)R;
+  const unsigned gb = grid_for(n);
+  int st;
+  // V[0] = v0 / ||v0||
+  SEEMPS_HIP(hipMemcpyAsync(w.V, v0, n * 16, hipMemcpyDeviceToDevice, st_));
+  if ((st = zmdotc_impl(n, 1, w.V, n, w.V, &w.sc->nrm, w.part, st_))) return st;
+  hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, w.V);
+  if ((st = check_launch("lz_unit"))) return st;
+  double theta = 0.0, resid = 0.0;
+  int64_t matvecs = 0, cycle = 0;
+  bool converged = false;
+  for (cycle = 0; cycle < maxiter; ++cycle) {
+    hipLaunchKernelGGL(lz_reset, dim3(1), dim3(64), 0, st_, w.sc);
+    for (int64_t j = 0; j < m; ++j) {
+      double2* vj = w.V + j * n;
+      double2* wv = w.V + (j + 1) * n;
+      if ((st = two_site_matvec_impl(chiL, c, chiL, d1, d2, w2, chiR, chiR, L_, H_, R_, vj, wv,
+                                     w.x1, w.x2, st_)))
+        return st;
+      ++matvecs;
+      // Two classical Gram-Schmidt passes against V[0..j].
+      if ((st = zmdotc_impl(n, j + 1, w.V, n, wv, w.sc->h, w.part, st_))) return st;
+      hipLaunchKernelGGL(maxpy_kernel, dim3(gb), dim3(256), 0, st_, n, (int)(j + 1), w.V, n,
+                         w.sc->h, wv);
+      if ((st = check_launch("maxpy"))) return st;
+      if ((st = zmdotc_impl(n, j + 1, w.V, n, wv, w.sc->h2, w.part, st_))) return st;
+      hipLaunchKernelGGL(maxpy_kernel, dim3(gb), dim3(256), 0, st_, n, (int)(j + 1), w.V, n,
+                         w.sc->h2, wv);
+      if ((st = check_launch("maxpy"))) return st;
+      if ((st = zmdotc_impl(n, 1, wv, n, wv, &w.sc->nrm, w.part, st_))) return st;
+      hipLaunchKernelGGL(lz_record, dim3(1), dim3(64), 0, st_, w.sc, (int)j);
+      hipLaunchKernelGGL(lz_normalize, dim3(gb), dim3(256), 0, st_, n, w.sc, (int)j, wv, wv);
+      if ((st = check_launch("lz_normalize"))) return st;
+    }
+    hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)m);
+    if ((st = check_launch("lz_tridiag"))) return st;
+    // Ritz vector into V[m], normalise, restart from it.
+    double2* x = w.V + m * n;
+    hipLaunchKernelGGL(combine_kernel, dim3(gb), dim3(256), 0, st_, n, (int)m, w.V, n, w.sc->y, x);
+    if ((st = check_launch("combine"))) return st;
+    if ((st = zmdotc_impl(n, 1, x, n, x, &w.sc->nrm, w.part, st_))) return st;
+    hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, x);
+    SEEMPS_HIP(hipMemcpyAsync(w.V, x, n * 16, hipMemcpyDeviceToDevice, st_));
+    double h[2];
+    SEEMPS_HIP(hipMemcpyAsync(&h[0], &w.sc->theta, 8, hipMemcpyDeviceToHost, st_));
+    SEEMPS_HIP(hipMemcpyAsync(&h[1], &w.sc->resid, 8, hipMemcpyDeviceToHost, st_));
+    SEEMPS_HIP(hipStreamSynchronize(st_));
+    theta = h[0];
+    resid = h[1];
+    if (resid <= tol * fmax(eps23, fabs(theta))) {
+      converged = true;
+      ++cycle;
+      break;
+    }
+  }
+  SEEMPS_HIP(hipMemcpyAsync(evec, w.V, n * 16, hipMemcpyDeviceToDevice, st_));
+  if (stats) {
+    stats[0] = theta;
+    stats[1] = resid;
+    stats[2] = (double)matvecs;
+    stats[3] = (double)cycle;
+  }
+  if (!converged) {
+    set_error("eigensolve: no convergence after %lld restarts (residual %g)", (long long)cycle,
+              resid);
+    return SEEMPS_ERR_NOCONVERGE;
+  }
+  return SEEMPS_OK;
+}

@@ -1,0 +1,776 @@
+// FP64 complex SVD (one-sided block Jacobi) and the Strategy truncation rule.
+//
+// Algorithm (Hestenes one-sided Jacobi, blocked):
+//   The working matrix W (ncol columns of length len; A itself when m >= n,
+//   A^H otherwise) is split into column blocks of width BW.  A sweep visits every
+//   pair of blocks once (round-robin "circle" schedule, nb/2 disjoint pairs per
+//   step).  For a pair, the 2BW x 2BW Gram matrix G = W_pq^H W_pq is formed,
+//   an inner cyclic Jacobi sweep on G accumulates a unitary Q, and the columns
+//   W_pq <- W_pq Q and V_pq <- V_pq Q are rotated.  Sweeps stop when every
+//   normalised off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in a sweep is below
+//   8 sqrt(len) eps.  Then sigma_k = ||w_k||, U = W / sigma, V accumulated.
+//
+// Parallel mapping: one cooperative persistent kernel for all sweeps.  Each step
+// has nb/2 pairs x R row chunks work items; phase A writes per-chunk partial
+// Grams, a grid barrier, phase B reduces them and runs the inner Jacobi sweep
+// (redundantly in each chunk's workgroup, bitwise identical), phase C rotates
+// its chunk of rows, a grid barrier.  Barrier waits are bounded: a workgroup
+// that spins too long raises an abort flag that every other waiter observes, so
+// all waves always exit.
+//
+// Replaces _destructive_svd (src/seemps/cython/svd.pxi:17-83) and the
+// truncation rules of src/seemps/cython/truncation.pxi:222-334.
+#include <cmath>
+
+#include "common.h"
+
+namespace seemps {
+namespace {
+
+constexpr int BW = 16;          // block width (columns)
+constexpr int PW = 2 * BW;      // pair width
+constexpr int RT = 32;          // rows per LDS tile
+constexpr int RTP = RT + 1;     // padded tile row length (complex)
+constexpr int RMAX = 16;        // max row chunks per pair
+constexpr int MAX_SWEEPS = 40;
+constexpr unsigned long long SPIN_LIMIT = 4000000ull;
+
+struct Ctrl {
+  unsigned int barrier;       // grid barrier arrivals
+  unsigned int abort;         // set when a barrier wait timed out
+  unsigned long long off[2];  // per-sweep max normalised off-diagonal (as bits)
+  int info[4];                // sweeps, converged, aborted, unused
+};
+
+struct SvdShape {
+  int64_t m, n;         // input shape
+  int64_t len, ncol;    // working matrix: ncol columns of length len
+  int64_t nb, ncolp;    // blocks (even), padded column count
+  int64_t P, R;         // pairs per step, row chunks
+  int transposed;       // 1 if W = A^H
+};
+
+struct SvdWork {
+  double2* W;     // ncolp x len
+  double2* V;     // ncolp x ncolp
+  double2* part;  // P x RMAX x PW x PW
+  double* sigma;  // ncolp
+  int* perm;      // ncolp
+  Ctrl* ctrl;
+};
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
+  SvdShape s;
+  s.m = m;
+  s.n = n;
+  s.transposed = (m < n);
+  s.len = s.transposed ? n : m;
+  s.ncol = s.transposed ? m : n;
+  int64_t nb = ceil_div(s.ncol, BW);
+  if (nb < 2) nb = 2;
+  nb += (nb & 1);
+  s.nb = nb;
+  s.ncolp = nb * BW;
+  s.P = nb / 2;
+  int64_t R = num_cu / s.P;
+  const int64_t rmax_len = ceil_div(s.len, RT);
+  if (R > rmax_len) R = rmax_len;
+  if (R > RMAX) R = RMAX;
+  if (R < 1) R = 1;
+  s.R = R;
+  return s;
+}
+
+int64_t svd_bytes(const SvdShape& s) {
+  int64_t b = 0;
+  b += align256(s.ncolp * s.len * 16);
+  b += align256(s.ncolp * s.ncolp * 16);
+  b += align256(s.P * RMAX * PW * PW * 16);
+  b += align256(s.ncolp * 8);
+  b += align256(s.ncolp * 4);
+  b += align256(sizeof(Ctrl));
+  return b;
+}
+
+SvdWork svd_carve(const SvdShape& s, void* work) {
+  char* p = (char*)work;
+  SvdWork w;
+  w.W = (double2*)p;
+  p += align256(s.ncolp * s.len * 16);
+  w.V = (double2*)p;
+  p += align256(s.ncolp * s.ncolp * 16);
+  w.part = (double2*)p;
+  p += align256(s.P * RMAX * PW * PW * 16);
+  w.sigma = (double*)p;
+  p += align256(s.ncolp * 8);
+  w.perm = (int*)p;
+  p += align256(s.ncolp * 4);
+  w.ctrl = (Ctrl*)p;
+  return w;
+}
+
+// W[c][r] = A[r][c] (m >= n) or conj(A[c][r]) (m < n); V = identity; ctrl = 0.
+__global__ void svd_prepare(SvdShape s, const double2* __restrict__ A, int64_t lda, SvdWork w) {
+  const int64_t totalW = s.ncolp * s.len;
+  const int64_t totalV = s.ncolp * s.ncolp;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < totalW; e += stride) {
+    const int64_t c = e / s.len, r = e % s.len;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < s.ncol) {
+      if (!s.transposed) {
+        v = A[r * lda + c];
+      } else {
+        v = A[c * lda + r];
+        v.y = -v.y;
+      }
+    }
+    w.W[e] = v;
+  }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < totalV; e += stride) {
+    const int64_t c = e / s.ncolp, r = e % s.ncolp;
+    w.V[e] = make_double2((c == r && c < s.ncol) ? 1.0 : 0.0, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.ctrl->barrier = 0;
+    w.ctrl->abort = 0;
+    w.ctrl->off[0] = 0;
+    w.ctrl->off[1] = 0;
+    w.ctrl->info[0] = w.ctrl->info[1] = w.ctrl->info[2] = w.ctrl->info[3] = 0;
+  }
+}
+
+// Bounded grid barrier.  Returns false if the barrier was aborted.
+__device__ bool grid_barrier(Ctrl* ctrl, unsigned int target, int* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(&ctrl->barrier, 1u);
+    unsigned long long spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(&ctrl->barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+           target) {
+      if (__hip_atomic_load(&ctrl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        ok = 0;
+        break;
+      }
+      if (++spins > SPIN_LIMIT) {
+        atomicExch(&ctrl->abort, 1u);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __threadfence();
+    *s_flag = ok;
+  }
+  __syncthreads();
+  return *s_flag != 0;
+}
+
+// Block indices of pair p in round k of the circle schedule over nb blocks.
+__device__ __forceinline__ void pair_blocks(int64_t nb, int64_t k, int64_t p, int64_t* bi,
+                                            int64_t* bj) {
+  auto idx = [&](int64_t x) { return 1 + (x + k) % (nb - 1); };
+  if (p == 0) {
+    *bi = 0;
+    *bj = idx(0);
+  } else {
+    *bi = idx(p);
+    *bj = idx(nb - 1 - p);
+  }
+}
+
+// Column index (in W / V) of local column t (0 <= t < PW) of the pair.
+__device__ __forceinline__ int64_t pair_col(int64_t bi, int64_t bj, int t) {
+  return (t < BW) ? bi * BW + t : bj * BW + (t - BW);
+}
+
+// Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS),
+// accumulating Q (LDS, on entry identity).  256 threads.
+__device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, int sweeps) {
+  const int tid = threadIdx.x;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int k = 0; k < PW - 1; ++k) {
+      // Pair t of round k over PW indices (circle schedule).
+      if (tid < PW / 2) {
+        const int t = tid;
+        int a, b;
+        auto idx = [&](int x) { return 1 + (x + k) % (PW - 1); };
+        if (t == 0) {
+          a = 0;
+          b = idx(0);
+        } else {
+          a = idx(t);
+          b = idx(PW - 1 - t);
+        }
+        const double gaa = G[a * PW + a].x, gbb = G[b * PW + b].x;
+        const double2 c = G[a * PW + b];
+        const double ac = sqrt(cabs2(c));
+        double cs = 1.0, sn = 0.0;
+        double2 ph = make_double2(1.0, 0.0);
+        if (ac > 0.0 && gaa > 0.0 && gbb > 0.0 && ac > tol * sqrt(gaa) * sqrt(gbb)) {
+          ph = make_double2(c.x / ac, c.y / ac);
+          const double tau = (gbb - gaa) / (2.0 * ac);
+          double t_;
+          if (fabs(tau) > 1e150)
+            t_ = 0.5 / tau;
+          else
+            t_ = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          cs = 1.0 / sqrt(1.0 + t_ * t_);
+          sn = t_ * cs;
+        }
+        rot[3 * t + 0] = make_double2(cs, sn);
+        rot[3 * t + 1] = ph;
+        rot[3 * t + 2] = make_double2((double)a, (double)b);
+      }
+      __syncthreads();
+      // Column pass: G <- G J, Q <- Q J, J = [[cs, sn e^{i phi}], [-sn e^{-i phi}, cs]].
+      for (int e = tid; e < 2 * PW * (PW / 2); e += blockDim.x) {
+        const int which = e / (PW * (PW / 2));  // 0: G, 1: Q
+        const int rem = e % (PW * (PW / 2));
+        const int t = rem / PW, x = rem % PW;
+        const double cs = rot[3 * t].x, sn = rot[3 * t].y;
+        if (sn == 0.0) continue;
+        const double2 ph = rot[3 * t + 1];
+        const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
+        double2* M = which ? Q : G;
+        const double2 xa = M[x * PW + a], xb = M[x * PW + b];
+        const double2 phc = make_double2(ph.x, -ph.y);
+        const double2 na = make_double2(cs * xa.x, cs * xa.y);
+        const double2 tb = cmul(phc, xb);
+        const double2 ta = cmul(ph, xa);
+        M[x * PW + a] = make_double2(na.x - sn * tb.x, na.y - sn * tb.y);
+        M[x * PW + b] = make_double2(sn * ta.x + cs * xb.x, sn * ta.y + cs * xb.y);
+      }
+      __syncthreads();
+      // Row pass: G <- J^H G.
+      for (int e = tid; e < PW * (PW / 2); e += blockDim.x) {
+        const int t = e / PW, y = e % PW;
+        const double cs = rot[3 * t].x, sn = rot[3 * t].y;
+        if (sn == 0.0) continue;
+        const double2 ph = rot[3 * t + 1];
+        const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
+        const double2 ay = G[a * PW + y], by = G[b * PW + y];
+        const double2 phc = make_double2(ph.x, -ph.y);
+        const double2 t1 = cmul(ph, by), t2 = cmul(phc, ay);
+        G[a * PW + y] = make_double2(cs * ay.x - sn * t1.x, cs * ay.y - sn * t1.y);
+        G[b * PW + y] = make_double2(sn * t2.x + cs * by.x, sn * t2.y + cs * by.y);
+      }
+      __syncthreads();
+      if (tid < PW / 2) {
+        const int t = tid;
+        if (rot[3 * t].y != 0.0) {
+          const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
+          G[a * PW + b] = make_double2(0.0, 0.0);
+          G[b * PW + a] = make_double2(0.0, 0.0);
+          G[a * PW + a].y = 0.0;
+          G[b * PW + b].y = 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Rotate rows [r0, r1) of the PW columns (bi, bj) of a column-major matrix M
+// (column stride ld) by Q: M_pq <- M_pq Q.  Uses an RT-row LDS tile.
+__device__ void rotate_rows(double2* __restrict__ M, int64_t ld, int64_t bi, int64_t bj,
+                            int64_t r0, int64_t r1, const double2* Q, double2* tile) {
+  const int tid = threadIdx.x;
+  for (int64_t rb = r0; rb < r1; rb += RT) {
+    const int nr = (int)min((int64_t)RT, r1 - rb);
+    for (int e = tid; e < PW * RT; e += blockDim.x) {
+      const int t = e / RT, rr = e % RT;
+      tile[t * RTP + rr] = (rr < nr) ? M[pair_col(bi, bj, t) * ld + rb + rr] : make_double2(0, 0);
+    }
+    __syncthreads();
+    for (int e = tid; e < PW * RT; e += blockDim.x) {
+      const int j = e / RT, rr = e % RT;
+      if (rr < nr) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll 8
+        for (int k = 0; k < PW; ++k) acc = cfma(tile[k * RTP + rr], Q[k * PW + j], acc);
+        M[pair_col(bi, bj, j) * ld + rb + rr] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double tol,
+                                                  int max_sweeps) {
+  __shared__ double2 tile[PW * RTP];
+  __shared__ double2 G[PW * PW];
+  __shared__ double2 Q[PW * PW];
+  __shared__ double2 rot[3 * (PW / 2)];
+  __shared__ int s_flag;
+  __shared__ double s_off[256];
+
+  const int tid = threadIdx.x;
+  const int64_t items = s.P * s.R;
+  const int64_t rc = ceil_div(s.len, s.R);      // rows of W per chunk
+  const int64_t rcv = ceil_div(s.ncolp, s.R);   // rows of V per chunk
+  const int inner_sweeps = (s.nb == 2) ? 3 : 1;
+  unsigned int barrier_no = 0;
+  const unsigned int G_ = gridDim.x;
+  int sweep = 0;
+  int converged = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    for (int64_t k = 0; k < s.nb - 1; ++k) {
+      // Phase A: partial Gram matrices.
+      for (int64_t it = blockIdx.x; it < items; it += G_) {
+        const int64_t p = it / s.R, r = it % s.R;
+        int64_t bi, bj;
+        pair_blocks(s.nb, k, p, &bi, &bj);
+        const int64_t r0 = r * rc, r1 = min(s.len, r0 + rc);
+        double2 acc[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0),
+                          make_double2(0, 0)};
+        for (int64_t rb = r0; rb < r1; rb += RT) {
+          const int nr = (int)min((int64_t)RT, r1 - rb);
+          for (int e = tid; e < PW * RT; e += blockDim.x) {
+            const int t = e / RT, rr = e % RT;
+            tile[t * RTP + rr] =
+                (rr < nr) ? w.W[pair_col(bi, bj, t) * s.len + rb + rr] : make_double2(0, 0);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int e = tid + u * 256;
+            const int i = e / PW, j = e % PW;
+            double2 a = acc[u];
+            for (int rr = 0; rr < nr; ++rr) a = cfmac(tile[i * RTP + rr], tile[j * RTP + rr], a);
+            acc[u] = a;
+          }
+          __syncthreads();
+        }
+        double2* dst = w.part + (p * RMAX + r) * PW * PW;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dst[tid + u * 256] = acc[u];
+      }
+      if (!grid_barrier(w.ctrl, (++barrier_no) * G_, &s_flag)) goto aborted;
+      // Every workgroup has read the previous sweep's flag (slot (sweep+1)&1)
+      // before arriving here: it can now be cleared for the next sweep.
+      if (k == 0 && blockIdx.x == 0 && tid == 0) w.ctrl->off[(sweep + 1) & 1] = 0ull;
+
+      // Phase B + C: reduce, inner Jacobi, rotate this chunk's rows.
+      for (int64_t it = blockIdx.x; it < items; it += G_) {
+        const int64_t p = it / s.R, r = it % s.R;
+        int64_t bi, bj;
+        pair_blocks(s.nb, k, p, &bi, &bj);
+        const double2* src = w.part + p * RMAX * PW * PW;
+        for (int e = tid; e < PW * PW; e += blockDim.x) {
+          double2 g = make_double2(0.0, 0.0);
+          for (int64_t q = 0; q < s.R; ++q) g = cadd(g, src[q * PW * PW + e]);
+          G[e] = g;
+          Q[e] = make_double2((e / PW == e % PW) ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+        if (r == 0) {
+          double mx = 0.0;
+          for (int e = tid; e < PW * PW; e += blockDim.x) {
+            const int i = e / PW, j = e % PW;
+            if (i < j) {
+              const double gi = G[i * PW + i].x, gj = G[j * PW + j].x;
+              if (gi > 0.0 && gj > 0.0) mx = fmax(mx, sqrt(cabs2(G[e])) / sqrt(gi) / sqrt(gj));
+            }
+          }
+          s_off[tid] = mx;
+          __syncthreads();
+          if (tid == 0) {
+            double m2 = 0.0;
+            for (int q = 0; q < (int)blockDim.x; ++q) m2 = fmax(m2, s_off[q]);
+            atomicMax(&w.ctrl->off[sweep & 1], (unsigned long long)__double_as_longlong(m2));
+          }
+          __syncthreads();
+        }
+        inner_jacobi(G, Q, rot, tol, inner_sweeps);
+        rotate_rows(w.W, s.len, bi, bj, r * rc, min(s.len, r * rc + rc), Q, tile);
+        rotate_rows(w.V, s.ncolp, bi, bj, r * rcv, min(s.ncolp, r * rcv + rcv), Q, tile);
+      }
+      if (!grid_barrier(w.ctrl, (++barrier_no) * G_, &s_flag)) goto aborted;
+    }
+    {
+      const unsigned long long bits =
+          __hip_atomic_load(&w.ctrl->off[sweep & 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      if (__longlong_as_double((long long)bits) <= tol) {
+        converged = 1;
+        ++sweep;
+        break;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    w.ctrl->info[0] = sweep;
+    w.ctrl->info[1] = converged;
+  }
+  return;
+aborted:
+  if (blockIdx.x == 0 && tid == 0) w.ctrl->info[2] = 1;
+}
+
+// sigma_c = ||w_c|| and descending ranks (ties by index).
+__global__ void svd_sigma(SvdShape s, SvdWork w) {
+  for (int64_t c = blockIdx.x; c < s.ncolp; c += gridDim.x) {
+    double acc = 0.0;
+    for (int64_t r = threadIdx.x; r < s.len; r += blockDim.x) acc += cabs2(w.W[c * s.len + r]);
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) w.sigma[c] = sqrt(red[0]);
+    __syncthreads();
+  }
+}
+
+__global__ void svd_rank(SvdShape s, SvdWork w) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncolp;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const double sc = w.sigma[c];
+    int rank = 0;
+    for (int64_t d = 0; d < s.ncolp; ++d) {
+      const double sd = w.sigma[d];
+      rank += (sd > sc || (sd == sc && d < c)) ? 1 : 0;
+    }
+    w.perm[rank] = (int)c;
+  }
+}
+
+// Write U (m x r), S (r), VH (r x n), r = ncol.
+__global__ void svd_output(SvdShape s, SvdWork w, double2* __restrict__ U, double* __restrict__ S,
+                           double2* __restrict__ VH) {
+  const int64_t r = s.ncol;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t k = t0; k < r; k += stride) S[k] = w.sigma[w.perm[k]];
+  // Side built from W (normalised columns): U when m >= n, VH when m < n.
+  for (int64_t e = t0; e < r * s.len; e += stride) {
+    const int64_t k = e / s.len, x = e % s.len;
+    const int64_t c = w.perm[k];
+    const double sg = w.sigma[c];
+    double2 v = w.W[c * s.len + x];
+    v = (sg > 0.0) ? make_double2(v.x / sg, v.y / sg) : make_double2(0.0, 0.0);
+    if (!s.transposed)
+      U[x * r + k] = v;  // U[x][k], m x r
+    else
+      VH[k * s.n + x] = make_double2(v.x, -v.y);  // VH[k][x] = conj(W_k[x])
+  }
+  // Side built from V: VH when m >= n, U when m < n.
+  for (int64_t e = t0; e < r * s.ncol; e += stride) {
+    const int64_t k = e / s.ncol, x = e % s.ncol;
+    const double2 v = w.V[(int64_t)w.perm[k] * s.ncolp + x];
+    if (!s.transposed)
+      VH[k * s.n + x] = make_double2(v.x, -v.y);
+    else
+      U[x * r + k] = v;
+  }
+}
+
+// Complete the W-side singular vectors of exactly-zero singular values to an
+// orthonormal set (classical Gram-Schmidt twice against unit candidates e_j).
+// One workgroup; only launched when such columns exist.
+__global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, double2* __restrict__ U,
+                                                    double2* __restrict__ VH) {
+  const int64_t r = s.ncol, len = s.len;
+  // Vector k of the W side: element x at X(k, x).
+  auto addr = [&](int64_t k, int64_t x) -> double2* {
+    return !s.transposed ? &U[x * r + k] : &VH[k * s.n + x];
+  };
+  __shared__ double2 red[256];
+  __shared__ double2 coef;
+  int64_t cand = 0;
+  for (int64_t k = 0; k < r; ++k) {
+    if (w.sigma[w.perm[k]] > 0.0) continue;
+    for (; cand < len; ++cand) {
+      for (int64_t x = threadIdx.x; x < len; x += blockDim.x)
+        *addr(k, x) = make_double2(x == cand ? 1.0 : 0.0, 0.0);
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t j = 0; j < k; ++j) {
+          double2 acc = make_double2(0.0, 0.0);
+          for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
+            double2 a = *addr(j, x), b = *addr(k, x);
+            if (s.transposed) {  // rows of VH are conjugated vectors
+              a.y = -a.y;
+              b.y = -b.y;
+            }
+            acc = cfmac(a, b, acc);
+          }
+          red[threadIdx.x] = acc;
+          __syncthreads();
+          if (threadIdx.x == 0) {
+            double2 t = make_double2(0, 0);
+            for (int q = 0; q < (int)blockDim.x; ++q) t = cadd(t, red[q]);
+            coef = t;
+          }
+          __syncthreads();
+          for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
+            double2 a = *addr(j, x), b = *addr(k, x);
+            if (s.transposed) {
+              a.y = -a.y;
+              b.y = -b.y;
+            }
+            const double2 pa = cmul(coef, a);
+            b = make_double2(b.x - pa.x, b.y - pa.y);
+            if (s.transposed) b.y = -b.y;
+            *addr(k, x) = b;
+          }
+          __syncthreads();
+        }
+      }
+      double acc = 0.0;
+      for (int64_t x = threadIdx.x; x < len; x += blockDim.x) acc += cabs2(*addr(k, x));
+      red[threadIdx.x] = make_double2(acc, 0.0);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0;
+        for (int q = 0; q < (int)blockDim.x; ++q) t += red[q].x;
+        coef = make_double2(sqrt(t), 0.0);
+      }
+      __syncthreads();
+      const double nrm = coef.x;
+      if (nrm > 0.5) {
+        for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
+          double2 b = *addr(k, x);
+          *addr(k, x) = make_double2(b.x / nrm, b.y / nrm);
+        }
+        __syncthreads();
+        ++cand;
+        break;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Truncation rule: one lane runs the reference's sequential loops over an LDS
+// copy of s (summation order identical to truncation.pxi:251-327).
+__global__ void truncate_kernel(int64_t N, const double* __restrict__ s, int method, double tol,
+                                int64_t max_bond, int64_t* D_out, double* err_out) {
+  extern __shared__ double ss[];
+  for (int64_t i = threadIdx.x; i < N; i += blockDim.x) ss[i] = s[i];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int64_t D = N;
+  double err = 0.0;
+  if (method == SEEMPS_TRUNC_NONE) {
+    D = N;
+    err = 0.0;
+  } else if (method == SEEMPS_TRUNC_REL_NORM2) {
+    // errors[i] = sum of the i smallest s^2, accumulated from the smallest.
+    double total = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+      const double x = ss[N - 1 - i];
+      total += x * x;
+    }
+    const double max_error = total * tol;
+    double acc = 0.0;
+    int64_t i_stop = N - 1;
+    for (int64_t i = 0; i < N; ++i) {
+      if (acc > max_error) {
+        i_stop = i - 1;
+        break;
+      }
+      const double x = ss[N - 1 - i];
+      acc += x * x;
+    }
+    D = N - i_stop;
+    if (D > max_bond) D = max_bond;
+    // err = errors[N - D]
+    double e = 0.0;
+    for (int64_t i = 0; i < N - D; ++i) {
+      const double x = ss[N - 1 - i];
+      e += x * x;
+    }
+    err = e;
+  } else {
+    const double thr = (method == SEEMPS_TRUNC_REL_SV) ? tol * ss[0] : tol;
+    D = (N < max_bond) ? N : max_bond;
+    for (int64_t i = 1; i < D; ++i) {
+      if (ss[i] <= thr) {
+        D = i;
+        break;
+      }
+    }
+    double e = 0.0;
+    for (int64_t i = D; i < N; ++i) e += ss[i] * ss[i];
+    err = e;
+  }
+  *D_out = D;
+  *err_out = err;
+}
+
+int cooperative_grid(int64_t items, int* grid) {
+  DeviceInfo di;
+  int st = device_info(&di);
+  if (st) return st;
+  static thread_local int occ_dev = -1, occ = 0;
+  if (occ_dev != di.device) {
+    int o = 0;
+    SEEMPS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)svd_jacobi, 256, 0));
+    occ = o;
+    occ_dev = di.device;
+  }
+  int64_t lim = (int64_t)occ * di.num_cu;
+  if (lim < 1) {
+    set_error("svd: kernel cannot be resident on this device");
+    return SEEMPS_ERR_RUNTIME;
+  }
+  int64_t g = items < lim ? items : lim;
+  if (g > 1024) g = 1024;
+  *grid = (int)g;
+  return SEEMPS_OK;
+}
+
+}  // namespace
+
+int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
+                double2* VH, void* work, int64_t work_bytes, hipStream_t stream, int* info_dev_off) {
+  DeviceInfo di;
+  int st = device_info(&di);
+  if (st) return st;
+  const SvdShape s = svd_shape(m, n, di.num_cu);
+  SEEMPS_CHECK_ARG(work_bytes >= svd_bytes(s), "svd: workspace too small (%lld < %lld)",
+                   (long long)work_bytes, (long long)svd_bytes(s));
+  SvdWork w = svd_carve(s, work);
+  int64_t blocks = ceil_div(s.ncolp * (s.len > s.ncolp ? s.len : s.ncolp), 256);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(svd_prepare, dim3((unsigned)blocks), dim3(256), 0, stream, s, A, lda, w);
+  if ((st = check_launch("svd_prepare"))) return st;
+  int grid = 1;
+  if ((st = cooperative_grid(s.P * s.R, &grid))) return st;
+  const double tol = 8.0 * sqrt((double)s.len) * 2.220446049250313e-16;
+  int max_sweeps = MAX_SWEEPS;
+  void* args[] = {(void*)&s, (void*)&w, (void*)&tol, (void*)&max_sweeps};
+  SEEMPS_HIP(hipLaunchCooperativeKernel((const void*)svd_jacobi, dim3(grid), dim3(256), args, 0,
+                                        stream));
+  hipLaunchKernelGGL(svd_sigma, dim3((unsigned)(s.ncolp < 1024 ? s.ncolp : 1024)), dim3(256), 0,
+                     stream, s, w);
+  if ((st = check_launch("svd_sigma"))) return st;
+  hipLaunchKernelGGL(svd_rank, dim3((unsigned)ceil_div(s.ncolp, 256)), dim3(256), 0, stream, s, w);
+  if ((st = check_launch("svd_rank"))) return st;
+  int64_t ob = ceil_div(s.ncol * (s.len > s.ncol ? s.len : s.ncol), 256);
+  if (ob > 2048) ob = 2048;
+  hipLaunchKernelGGL(svd_output, dim3((unsigned)ob), dim3(256), 0, stream, s, w, U, S, VH);
+  if ((st = check_launch("svd_output"))) return st;
+  hipLaunchKernelGGL(svd_complete, dim3(1), dim3(256), 0, stream, s, w, U, VH);
+  if ((st = check_launch("svd_complete"))) return st;
+  if (info_dev_off) *info_dev_off = (int)((char*)&w.ctrl->info[0] - (char*)work);
+  return SEEMPS_OK;
+}
+
+int64_t svd_workspace_bytes(int64_t m, int64_t n) {
+  // Sized for the largest row split so the size does not depend on the device.
+  SvdShape s = svd_shape(m, n, 1 << 20);
+  return svd_bytes(s);
+}
+
+int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t max_bond,
+                     int64_t* D_out, double* err_out, hipStream_t stream) {
+  SEEMPS_CHECK_ARG(N >= 1, "truncate: empty spectrum");
+  SEEMPS_CHECK_ARG(N <= 4096, "truncate: spectrum longer than 4096");
+  SEEMPS_CHECK_ARG(method >= 0 && method <= 3, "truncate: invalid method %d", method);
+  SEEMPS_CHECK_ARG(max_bond >= 1, "truncate: invalid max bond dimension");
+  hipLaunchKernelGGL(truncate_kernel, dim3(1), dim3(256), (size_t)N * sizeof(double), stream, N,
+                     s, method, tol, max_bond, D_out, err_out);
+  return check_launch("truncate");
+}
+
+}  // namespace seemps
+
+extern "C" int64_t seemps_svd_workspace(int64_t m, int64_t n) {
+  return seemps::svd_workspace_bytes(m, n);
+}
+
+static int read_info(void* work, int off, int32_t* info, hipStream_t stream) {
+  int32_t h[4] = {0, 0, 0, 0};
+  SEEMPS_HIP(hipMemcpyAsync(h, (char*)work + off, sizeof(h), hipMemcpyDeviceToHost, stream));
+  SEEMPS_HIP(hipStreamSynchronize(stream));
+  if (info) {
+    info[0] = h[0];
+    info[1] = h[1];
+  }
+  if (h[2]) {
+    seemps::set_error("svd: grid barrier timed out (cooperative residency lost)");
+    return SEEMPS_ERR_RUNTIME;
+  }
+  if (!h[1]) {
+    seemps::set_error("SVD did not converge in %d sweeps", h[0]);
+    return SEEMPS_ERR_NOCONVERGE;
+  }
+  return SEEMPS_OK;
+}
+
+extern "C" int seemps_zgesvd(int64_t m, int64_t n, const void* A, int64_t lda, void* U, double* S,
+                             void* VH, void* work, int64_t work_bytes, int32_t* info,
+                             void* stream) {
+  SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd: empty matrix");
+  SEEMPS_CHECK_ARG(lda >= n, "svd: lda < n");
+  int off = 0;
+  int st = seemps::svd_enqueue(m, n, (const double2*)A, lda, (double2*)U, S, (double2*)VH, work,
+                               work_bytes, (hipStream_t)stream, &off);
+  if (st) return st;
+  return read_info(work, off, info, (hipStream_t)stream);
+}
+
+extern "C" int seemps_truncate(int64_t N, const double* s, int method, double tolerance,
+                               int64_t max_bond, int64_t* D_out, double* err_out, void* stream) {
+  return seemps::truncate_enqueue(N, s, method, tolerance, max_bond, D_out, err_out,
+                                  (hipStream_t)stream);
+}
+
+extern "C" int64_t seemps_svd_split_workspace(int64_t m, int64_t n) {
+  const int64_t r = m < n ? m : n;
+  auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  return al(m * r * 16) + al(r * 8) + al(r * n * 16) + al(64) + seemps::svd_workspace_bytes(m, n);
+}
+
+extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, int method,
+                                double tolerance, int64_t max_bond, void* outA, void* outB,
+                                double* s_out, int64_t* D_host, double* err_host, void* work,
+                                int64_t work_bytes, int32_t* info, void* stream) {
+  SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd_split: empty matrix");
+  SEEMPS_CHECK_ARG(mode == 0 || mode == 1, "svd_split: invalid mode");
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_workspace(m, n), "svd_split: workspace too small");
+  hipStream_t st_ = (hipStream_t)stream;
+  const int64_t r = m < n ? m : n;
+  auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  char* p = (char*)work;
+  double2* U = (double2*)p;
+  p += al(m * r * 16);
+  double* S = (double*)p;
+  p += al(r * 8);
+  double2* VH = (double2*)p;
+  p += al(r * n * 16);
+  int64_t* Ddev = (int64_t*)p;
+  double* edev = (double*)(p + 8);
+  p += al(64);
+  int off = 0;
+  int st = seemps::svd_enqueue(m, n, (const double2*)A, n, U, S, VH, p,
+                               work_bytes - (p - (char*)work), st_, &off);
+  if (st) return st;
+  if ((st = seemps::truncate_enqueue(r, S, method, tolerance, max_bond, Ddev, edev, st_)))
+    return st;
+  int64_t hD = 0;
+  double herr = 0.0;
+  SEEMPS_HIP(hipMemcpyAsync(&hD, Ddev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(&herr, edev, 8, hipMemcpyDeviceToHost, st_));
+  if ((st = read_info(p, off, info, st_))) return st;
+  const int64_t D = hD;
+  if (mode == 0) {
+    if ((st = seemps::zscale2d_impl(m, D, nullptr, nullptr, U, r, (double2*)outA, D, st_))) return st;
+    if ((st = seemps::zscale2d_impl(D, n, S, nullptr, VH, n, (double2*)outB, n, st_))) return st;
+  } else {
+    if ((st = seemps::zscale2d_impl(m, D, nullptr, S, U, r, (double2*)outA, D, st_))) return st;
+    if ((st = seemps::zscale2d_impl(D, n, nullptr, nullptr, VH, n, (double2*)outB, n, st_))) return st;
+  }
+  if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, S, r * 8, hipMemcpyDeviceToDevice, st_));
+  *D_host = D;
+  *err_host = herr;
+  return SEEMPS_OK;
+}

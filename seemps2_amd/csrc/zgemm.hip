@@ -1,0 +1,256 @@
+// Complex128 GEMM on the FP64 matrix cores of gfx950 (v_mfma_f64_16x16x4f64).
+//
+// C = alpha * op(A) @ op(B) + beta * C for strided batches, row-major storage.
+// A complex product is four real MFMA products ("4M"): re += ar*br - ai*bi,
+// im += ar*bi + ai*br.  Operand tiles are staged in LDS split into real and
+// imaginary planes, K-major, so each MFMA fragment read is a contiguous run
+// of 16 doubles; the op(.) transposition / conjugation is folded into the
+// global->LDS load (no transposed copies are materialised).
+//
+// Tile: 64 x 64 outputs per 256-thread workgroup (4 waves of 32 x 32), K step
+// 16, next K-tile prefetched into registers while the current one is used.
+//
+// MFMA f64 16x16x4 lane layout (checked on the device): A[i][k] from lane
+// i + 16k, B[k][j] from lane j + 16k, result item r of lane l is D[l/16 + 4r][l%16].
+#include "common.h"
+
+namespace seemps {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+
+// Load the (BM x BK) tile of op(A) starting at (m0, k0) into registers: 4 complex
+// per thread.  For op in {N, C} A is stored M x K; for {T, H} it is stored K x M.
+template <int OP>
+__device__ __forceinline__ void load_a(const double2* __restrict__ A, int64_t lda, int64_t M,
+                                       int64_t K, int64_t m0, int64_t k0, double2 (&r)[4]) {
+  const int t = threadIdx.x;
+  if (OP == SEEMPS_OP_N || OP == SEEMPS_OP_C) {
+    const int64_t m = m0 + (t >> 2);
+    const int64_t kb = k0 + (t & 3) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = kb + u;
+      r[u] = (m < M && k < K) ? A[m * lda + k] : make_double2(0.0, 0.0);
+    }
+  } else {
+    const int64_t k = k0 + (t >> 4);
+    const int64_t mb = m0 + (t & 15) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t m = mb + u;
+      r[u] = (m < M && k < K) ? A[k * lda + m] : make_double2(0.0, 0.0);
+    }
+  }
+}
+
+template <int OP>
+__device__ __forceinline__ void store_a(double* __restrict__ s_re, double* __restrict__ s_im,
+                                        const double2 (&r)[4]) {
+  const int t = threadIdx.x;
+  const double sg = (OP == SEEMPS_OP_C || OP == SEEMPS_OP_H) ? -1.0 : 1.0;
+  if (OP == SEEMPS_OP_N || OP == SEEMPS_OP_C) {
+    const int m = t >> 2, kb = (t & 3) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s_re[(kb + u) * LDA_S + m] = r[u].x;
+      s_im[(kb + u) * LDA_S + m] = sg * r[u].y;
+    }
+  } else {
+    const int k = t >> 4, mb = (t & 15) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s_re[k * LDA_S + mb + u] = r[u].x;
+      s_im[k * LDA_S + mb + u] = sg * r[u].y;
+    }
+  }
+}
+
+// (BK x BN) tile of op(B) at (k0, n0).  For op in {N, C} B is stored K x N;
+// for {T, H} it is stored N x K.
+template <int OP>
+__device__ __forceinline__ void load_b(const double2* __restrict__ B, int64_t ldb, int64_t K,
+                                       int64_t N, int64_t k0, int64_t n0, double2 (&r)[4]) {
+  const int t = threadIdx.x;
+  if (OP == SEEMPS_OP_N || OP == SEEMPS_OP_C) {
+    const int64_t k = k0 + (t >> 4);
+    const int64_t nb = n0 + (t & 15) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t n = nb + u;
+      r[u] = (k < K && n < N) ? B[k * ldb + n] : make_double2(0.0, 0.0);
+    }
+  } else {
+    const int64_t n = n0 + (t >> 2);
+    const int64_t kb = k0 + (t & 3) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = kb + u;
+      r[u] = (k < K && n < N) ? B[n * ldb + k] : make_double2(0.0, 0.0);
+    }
+  }
+}
+
+template <int OP>
+__device__ __forceinline__ void store_b(double* __restrict__ s_re, double* __restrict__ s_im,
+                                        const double2 (&r)[4]) {
+  const int t = threadIdx.x;
+  const double sg = (OP == SEEMPS_OP_C || OP == SEEMPS_OP_H) ? -1.0 : 1.0;
+  if (OP == SEEMPS_OP_N || OP == SEEMPS_OP_C) {
+    const int k = t >> 4, nb = (t & 15) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s_re[k * LDB_S + nb + u] = r[u].x;
+      s_im[k * LDB_S + nb + u] = sg * r[u].y;
+    }
+  } else {
+    const int n = t >> 2, kb = (t & 3) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s_re[(kb + u) * LDB_S + n] = r[u].x;
+      s_im[(kb + u) * LDB_S + n] = sg * r[u].y;
+    }
+  }
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void zgemm_kernel(
+    int64_t M, int64_t N, int64_t K, double2 alpha, const double2* __restrict__ A, int64_t lda,
+    int64_t sA, const double2* __restrict__ B, int64_t ldb, int64_t sB, double2 beta,
+    double2* __restrict__ C, int64_t ldc, int64_t sC) {
+  __shared__ double as_re[BK * LDA_S], as_im[BK * LDA_S];
+  __shared__ double bs_re[BK * LDB_S], bs_im[BK * LDB_S];
+
+  const int64_t bz = blockIdx.z;
+  A += bz * sA;
+  B += bz * sB;
+  C += bz * sC;
+  const int64_t m0 = (int64_t)blockIdx.y * BM;
+  const int64_t n0 = (int64_t)blockIdx.x * BN;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+
+  d4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  double2 ra[4], rb[4];
+  load_a<OPA>(A, lda, M, K, m0, 0, ra);
+  load_b<OPB>(B, ldb, K, N, 0, n0, rb);
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    store_a<OPA>(as_re, as_im, ra);
+    store_b<OPB>(bs_re, bs_im, rb);
+    __syncthreads();
+    if (k0 + BK < K) {
+      load_a<OPA>(A, lda, M, K, m0, k0 + BK, ra);
+      load_b<OPB>(B, ldb, K, N, k0 + BK, n0, rb);
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int k = ks * 4 + lk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        ar[a] = as_re[k * LDA_S + wm + a * 16 + li];
+        ai[a] = as_im[k * LDA_S + wm + a * 16 + li];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        br[b] = bs_re[k * LDB_S + wn + b * 16 + li];
+        bi[b] = bs_im[k * LDB_S + wn + b * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
+          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + wm + a * 16 + lk + 4 * r;
+        const int64_t n = n0 + wn + b * 16 + li;
+        if (m < M && n < N) {
+          const double2 acc = make_double2(cr[a][b][r], ci[a][b][r]);
+          double2 out = cmul(alpha, acc);
+          if (use_beta) out = cfma(beta, C[m * ldc + n], out);
+          C[m * ldc + n] = out;
+        }
+      }
+}
+
+template <int OPA, int OPB>
+void launch(dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K, double2 alpha,
+            const double2* A, int64_t lda, int64_t sA, const double2* B, int64_t ldb, int64_t sB,
+            double2 beta, double2* C, int64_t ldc, int64_t sC) {
+  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA,
+                     B, ldb, sB, beta, C, ldc, sC);
+}
+
+}  // namespace
+
+// Enqueue a complex GEMM (shared by the other host entry points of the library).
+int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
+          int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB, double2 beta, void* C,
+          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream) {
+  SEEMPS_CHECK_ARG(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "zgemm: invalid op flags");
+  SEEMPS_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative dimension");
+  if (M == 0 || N == 0 || batch == 0) return SEEMPS_OK;
+  SEEMPS_CHECK_ARG(batch <= 65535, "zgemm: batch too large");
+  SEEMPS_CHECK_ARG(ceil_div(M, BM) <= 65535, "zgemm: M too large");
+  if (K == 0) {
+    // Pure scaling of C by beta (or zero fill).
+    SEEMPS_CHECK_ARG(ldc == N && (batch == 1 || sC == M * N), "zgemm: K == 0 needs packed C");
+    return zaxpby_impl(M * N * batch, make_double2(0.0, 0.0), nullptr, 0, beta, (double2*)C,
+                       stream);
+  }
+  dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)batch);
+  const double2* a = (const double2*)A;
+  const double2* b = (const double2*)B;
+  double2* c = (double2*)C;
+#define SEEMPS_GEMM_CASE(x, y)                                                              \
+  if (opA == x && opB == y) {                                                               \
+    launch<x, y>(grid, stream, M, N, K, alpha, a, lda, sA, b, ldb, sB, beta, c, ldc, sC); \
+    return check_launch("zgemm");                                                           \
+  }
+  SEEMPS_GEMM_CASE(0, 0) SEEMPS_GEMM_CASE(0, 1) SEEMPS_GEMM_CASE(0, 2) SEEMPS_GEMM_CASE(0, 3)
+  SEEMPS_GEMM_CASE(1, 0) SEEMPS_GEMM_CASE(1, 1) SEEMPS_GEMM_CASE(1, 2) SEEMPS_GEMM_CASE(1, 3)
+  SEEMPS_GEMM_CASE(2, 0) SEEMPS_GEMM_CASE(2, 1) SEEMPS_GEMM_CASE(2, 2) SEEMPS_GEMM_CASE(2, 3)
+  SEEMPS_GEMM_CASE(3, 0) SEEMPS_GEMM_CASE(3, 1) SEEMPS_GEMM_CASE(3, 2) SEEMPS_GEMM_CASE(3, 3)
+#undef SEEMPS_GEMM_CASE
+  set_error("zgemm: unreachable op combination");
+  return SEEMPS_ERR_INVALID;
+}
+
+}  // namespace seemps
+
+extern "C" int seemps_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double alpha_re,
+                            double alpha_im, const void* A, int64_t lda, int64_t strideA,
+                            const void* B, int64_t ldb, int64_t strideB, double beta_re,
+                            double beta_im, void* C, int64_t ldc, int64_t strideC,
+                            int64_t batch, void* stream) {
+  return seemps::zgemm(opA, opB, M, N, K, make_double2(alpha_re, alpha_im), A, lda, strideA, B,
+                       ldb, strideB, make_double2(beta_re, beta_im), C, ldc, strideC, batch,
+                       (hipStream_t)stream);
+}

@@ -1,0 +1,152 @@
+"""Parity of the raw C-ABI kernels against NumPy / the oracle on the device."""
+
+import numpy as np
+import pytest
+
+from oracle import core as ocore
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def K():
+    from seemps2_amd import kernels
+
+    return kernels
+
+
+def crandn(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+OPS = {0: lambda x: x, 1: lambda x: x.T, 2: lambda x: x.conj().T, 3: lambda x: x.conj()}
+
+
+@pytest.mark.parametrize("opA", [0, 1, 2, 3])
+@pytest.mark.parametrize("opB", [0, 1, 2, 3])
+@pytest.mark.parametrize("M,N,K_", [(37, 53, 29), (130, 70, 200), (1, 1, 1), (64, 64, 16)])
+def test_zgemm_ops(K, rng, opA, opB, M, N, K_):
+    A = crandn(rng, *((K_, M) if opA in (1, 2) else (M, K_)))
+    B = crandn(rng, *((N, K_) if opB in (1, 2) else (K_, N)))
+    C = crandn(rng, M, N)
+    alpha, beta = 0.7 - 0.2j, -0.3 + 1.1j
+    ref = alpha * OPS[opA](A) @ OPS[opB](B) + beta * C
+    Cd = K.as_dev(C)
+    out = K.gemm(K.as_dev(A), K.as_dev(B), opA, opB, alpha, beta, Cd)
+    assert rel(K.to_numpy(out), ref) < 1e-14
+
+
+def test_zlocal(K, rng):
+    for (P, I, Q, J1, J2) in [(7, 20, 130, 20, 1), (33, 2, 40, 10, 5), (1, 1, 1, 1, 1), (5, 10, 64, 10, 1)]:
+        x = crandn(rng, P, I, Q)
+        Mop = crandn(rng, I, J1, J2)
+        ref = np.einsum("piq,ijk->pjqk", x, Mop)
+        out = K.local(K.as_dev(x), K.as_dev(Mop), P, I, Q, J1, J2)
+        assert rel(K.to_numpy(out), ref) < 1e-14
+        out2 = K.local(K.as_dev(x), K.as_dev(Mop), P, I, Q, J1, J2, out=out, accumulate=True)
+        assert rel(K.to_numpy(out2), 2 * ref) < 1e-14
+
+
+def test_zdotc(K, rng):
+    for n in [1, 17, 4096, 100003]:
+        x, y = crandn(rng, n), crandn(rng, n)
+        assert abs(K.dotc(K.as_dev(x), K.as_dev(y)) - np.vdot(x, y)) < 1e-12 * np.sqrt(n) * 4
+
+
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+def test_truncate_matches_oracle(K, rng, method):
+    for trial in range(20):
+        N = int(rng.integers(1, 300))
+        s = np.sort(np.exp(rng.normal(scale=6.0, size=N)))[::-1]
+        tol = float(10.0 ** rng.uniform(-14, -1))
+        mb = int(rng.integers(1, N + 2))
+        st = ocore.Strategy(method=method, tolerance=tol, max_bond_dimension=mb)
+        D_ref, err_ref = ocore.truncate(s, st)
+        D, err = K.truncate(torch.tensor(s, device="cuda"), st.method, st.tolerance,
+                            st.max_bond_dimension)
+        assert D == D_ref
+        assert err == err_ref  # same summation order: bit-exact
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (3, 5), (16, 16), (17, 40), (64, 64),
+                                 (128, 96), (200, 300), (512, 512), (320, 160)])
+def test_svd_random(K, rng, m, n):
+    A = crandn(rng, m, n)
+    U, S, VH = K.svd(K.as_dev(A))
+    U, S, VH = K.to_numpy(U), S.cpu().numpy(), K.to_numpy(VH)
+    r = min(m, n)
+    assert rel((U * S) @ VH, A) < 1e-13
+    assert np.abs(U.conj().T @ U - np.eye(r)).max() < 1e-13
+    assert np.abs(VH @ VH.conj().T - np.eye(r)).max() < 1e-13
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(S - s_ref).max() < 1e-13 * s_ref[0]
+    assert np.all(np.diff(S) <= 0)
+
+
+def test_svd_rank_deficient_and_zero(K, rng):
+    A = crandn(rng, 20, 3) @ crandn(rng, 3, 20)
+    A[:, 5] = 0
+    for M in [A, A.T.copy(), np.zeros((6, 4)), np.zeros((4, 6))]:
+        U, S, VH = K.svd(K.as_dev(M))
+        U, S, VH = K.to_numpy(U), S.cpu().numpy(), K.to_numpy(VH)
+        r = min(M.shape)
+        assert rel((U * S) @ VH, M + 0j) < 1e-13 or np.linalg.norm(M) == 0
+        assert np.abs(U.conj().T @ U - np.eye(r)).max() < 1e-12
+        assert np.abs(VH @ VH.conj().T - np.eye(r)).max() < 1e-12
+
+
+def test_svd_split_matches_oracle(K, rng):
+    for (a, d, b), tol, mb in [((8, 2, 8), 1e-3, 100), ((30, 2, 30), 1e-10, 20), ((1, 2, 1), 1e-16, 5)]:
+        AA = crandn(rng, a, d, d, b)
+        AA /= np.linalg.norm(AA)
+        st = ocore.Strategy(tolerance=tol, max_bond_dimension=mb)
+        for mode, fn in [(0, ocore.left_orth_2site), (1, ocore.right_orth_2site)]:
+            Ar, Br, er = fn(AA, st)
+            Ad, Bd, D, err2, _ = K.svd_split(K.as_dev(AA), a * d, d * b, mode, st.method,
+                                             st.tolerance, st.max_bond_dimension)
+            assert D == Ar.shape[-1]
+            assert abs(np.sqrt(err2) - er) <= 1e-12 * max(1.0, er)
+            rec = K.to_numpy(Ad) @ K.to_numpy(Bd)
+            assert rel(rec, Ar.reshape(a * d, -1) @ Br.reshape(D, -1)) < 1e-12
+
+
+def test_two_site_matvec_and_eigensolve(K, rng):
+    from oracle import algorithms as oal, builders as obl
+
+    N, d = 6, 2
+    H = obl.xxz_mpo(N)
+    psi = obl.random_mps([d] * N, D=7, complex=True, rng=rng)
+    T = psi.tensors
+    # physical environments around the bond (2, 3)
+    L = np.ones((1, 1, 1))
+    for i in range(2):
+        L = oal.update_left_mpo_environment(L, T[i], H[i], T[i])
+    R = np.ones((1, 1, 1))
+    for i in range(N - 1, 3, -1):
+        R = oal.update_right_mpo_environment(R, T[i], H[i], T[i])
+    O1, O2 = H[2], H[3]
+    v = np.tensordot(T[2], T[3], axes=1)
+    H12 = np.tensordot(O1, O2, axes=1)
+    ref = oal.two_site_matvec(L, H12, R, v)
+    Hm = K.two_site_operator(K.as_dev(O1), K.as_dev(O2))
+    out = K.two_site_matvec(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d)
+    assert rel(K.to_numpy(out), ref) < 1e-13
+    n = v.size
+    Hd = np.zeros((n, n), complex)
+    for k in range(n):
+        ek = np.zeros(n, complex)
+        ek[k] = 1
+        Hd[:, k] = oal.two_site_matvec(L, H12, R, ek.reshape(v.shape)).reshape(-1)
+    assert np.abs(Hd - Hd.conj().T).max() < 1e-10 * np.abs(Hd).max()
+    w, V = np.linalg.eigh(Hd)
+    lam, vec, stats = K.eigensolve(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d, ncv=20,
+                                   tol=1e-12)
+    assert abs(lam - w[0]) < 1e-9 * max(1, abs(w[0]))
+    x = K.to_numpy(vec).reshape(-1)
+    assert abs(abs(np.vdot(V[:, 0], x)) - 1) < 1e-8
