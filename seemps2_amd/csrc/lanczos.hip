@@ -101,7 +101,126 @@ struct LzScalars {
   double scale;     // running max |alpha| + |beta| (breakdown reference)
   int kd;           // effective Krylov dimension of the cycle
   int broken;       // breakdown seen in this cycle
+  double scl[64];   // scale after step j
+  int brk[64];      // breakdown seen up to step j
 };
+
+constexpr int LZ_NV = 64;  // stride of the per-block partial dot products
+
+// Block-wide sum of one complex value per thread (256 threads), result valid
+// in every thread.
+__device__ double2 lz_block_sum(double2 v, double2* red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double2 t = make_double2(0.0, 0.0);
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t = cadd(t, red[q]);
+  return t;
+}
+
+// part[b * LZ_NV + j] = sum over chunk b of conj(V_j) w, j < k.
+__global__ __launch_bounds__(256) void lz_dots(int64_t n, int k, const double2* __restrict__ V,
+                                               const double2* __restrict__ w,
+                                               double2* __restrict__ part) {
+  __shared__ double2 red[4];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  for (int j = 0; j < k; ++j) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc = cfmac(V[j * n + i], w[i], acc);
+    const double2 t = lz_block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x * LZ_NV + j] = t;
+  }
+}
+
+// h = sum_b part_in (fixed order, in every block); w -= V h over this block's
+// chunk; then either the next dots (part_out, want_norm = 0) or the squared
+// norm (pnorm, want_norm = 1) of the updated chunk.  Block 0 stores h.
+__global__ __launch_bounds__(256) void lz_sub(int64_t n, int k, const double2* __restrict__ V,
+                                              double2* __restrict__ w,
+                                              const double2* __restrict__ part_in,
+                                              double2* __restrict__ part_out,
+                                              double* __restrict__ pnorm, double2* __restrict__ h_out,
+                                              int want_norm) {
+  __shared__ double2 hs[64];
+  __shared__ double2 red[4];
+  const int nb = gridDim.x;
+  if ((int)threadIdx.x < k) {
+    double2 t = make_double2(0.0, 0.0);
+    for (int b = 0; b < nb; ++b) t = cadd(t, part_in[b * LZ_NV + threadIdx.x]);
+    hs[threadIdx.x] = t;
+    if (blockIdx.x == 0) h_out[threadIdx.x] = t;
+  }
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    double2 acc = w[i];
+    for (int j = 0; j < k; ++j) {
+      const double2 t = cmul(V[j * n + i], hs[j]);
+      acc.x -= t.x;
+      acc.y -= t.y;
+    }
+    w[i] = acc;
+  }
+  if (want_norm) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc.x += cabs2(w[i]);
+    const double2 t = lz_block_sum(acc, red);
+    if (threadIdx.x == 0) pnorm[blockIdx.x] = t.x;
+  } else {
+    for (int j = 0; j < k; ++j) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+        acc = cfmac(V[j * n + i], w[i], acc);
+      const double2 t = lz_block_sum(acc, red);
+      if (threadIdx.x == 0) part_out[blockIdx.x * LZ_NV + j] = t;
+    }
+  }
+}
+
+// Step j done: beta_j = ||w|| (from the partial norms, in every block),
+// alpha_j = Re(h_j + h2_j), breakdown test, v_{j+1} = w / beta_j (0 after a
+// breakdown).  Block 0 records the scalars.
+__global__ __launch_bounds__(256) void lz_finish(int64_t n, int nparts, int j,
+                                                 const double* __restrict__ pnorm,
+                                                 LzScalars* __restrict__ s, double2* __restrict__ w) {
+  __shared__ double s_b;
+  __shared__ int s_brk;
+  if (threadIdx.x == 0) {
+    double nrm2 = 0.0;
+    for (int b = 0; b < nparts; ++b) nrm2 += pnorm[b];
+    const double a = s->h[j].x + s->h2[j].x;
+    const double b = sqrt(fmax(nrm2, 0.0));
+    const double prev = (j > 0) ? s->beta[j - 1] : 0.0;
+    const double scale_prev = (j > 0) ? s->scl[j - 1] : 0.0;
+    const int brk_prev = (j > 0) ? s->brk[j - 1] : 0;
+    const double ref = fmax(scale_prev, fabs(a) + prev);
+    const int brk = brk_prev || (b <= 1e-12 * ref);
+    s_b = b;
+    s_brk = brk;
+    if (blockIdx.x == 0) {
+      s->alpha[j] = a;
+      s->beta[j] = brk ? 0.0 : b;
+      s->scl[j] = ref;
+      s->brk[j] = brk;
+      s->broken = brk;
+      if (brk && !brk_prev) s->kd = j + 1;
+    }
+  }
+  __syncthreads();
+  const double inv = s_brk ? 0.0 : 1.0 / s_b;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = w[i];
+    w[i] = make_double2(x.x * inv, x.y * inv);
+  }
+}
 
 // After the two orthogonalisation passes of step j: record alpha_j, beta_j.
 __global__ void lz_record(LzScalars* s, int j) {
@@ -281,10 +400,21 @@ struct EigWork {
   double2* x1;     // a*c*d1*d2*f
   double2* x2;     // a*d1*d2*w2*f
   double2* part;   // reduction partials (64 x nblk)
+  double2* part2;  // second-pass partials (64 x nblk)
+  double* pnorm;   // partial squared norms (nblk)
+  double2* ks;     // split-K partials of the matvec GEMMs
   LzScalars* sc;
 };
 
 inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+// Split-K scratch (elements) of the two matvec GEMMs.
+int64_t matvec_ks_elems(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2, int64_t w2,
+                        int64_t f, int64_t g) {
+  const int64_t k1 = splitk_workspace(a * c, d1 * d2 * f, b);
+  const int64_t k2 = splitk_workspace(a * d1 * d2, g, w2 * f);
+  return k1 > k2 ? k1 : k2;
+}
 
 int64_t eig_bytes_full(int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t w2, int64_t chiR,
                        int64_t m) {
@@ -294,6 +424,9 @@ int64_t eig_bytes_full(int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t 
   b += al256(chiL * c * d1 * d2 * chiR * 16);
   b += al256(chiL * d1 * d2 * w2 * chiR * 16);
   b += al256(64 * reduce_blocks(n) * 16);
+  b += al256(64 * reduce_blocks(n) * 16);
+  b += al256(reduce_blocks(n) * 8);
+  b += al256(matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR) * 16);
   b += al256(sizeof(LzScalars));
   return b;
 }
@@ -311,6 +444,12 @@ EigWork eig_carve(void* work, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
   p += al256(chiL * d1 * d2 * w2 * chiR * 16);
   w.part = (double2*)p;
   p += al256(64 * reduce_blocks(n) * 16);
+  w.part2 = (double2*)p;
+  p += al256(64 * reduce_blocks(n) * 16);
+  w.pnorm = (double*)p;
+  p += al256(reduce_blocks(n) * 8);
+  w.ks = (double2*)p;
+  p += al256(matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR) * 16);
   w.sc = (LzScalars*)p;
   return w;
 }
@@ -328,18 +467,18 @@ unsigned grid_for(int64_t n) {
 int two_site_matvec_impl(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2, int64_t w2,
                          int64_t f, int64_t g, const double2* L, const double2* Hm,
                          const double2* R, const double2* v, double2* out, double2* x1,
-                         double2* x2, hipStream_t stream) {
+                         double2* x2, double2* ks, int64_t ks_elems, hipStream_t stream) {
   int st;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   // x1[(a,c), (k,l,f)] = L[(a,c), b] v[b, (k,l,f)]
-  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, a * c, d1 * d2 * f, b, one, L, b, 0, v, d1 * d2 * f,
-                  0, zero, x1, d1 * d2 * f, 0, 1, stream)))
+  if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, a * c, d1 * d2 * f, b, one, L, b, v, d1 * d2 * f,
+                     zero, x1, d1 * d2 * f, ks, ks_elems, stream)))
     return st;
   // x2[a, (ko,lo,d), f] = sum_{(c,k,l)} x1[a, (c,k,l), f] Hm[(c,k,l), (ko,lo,d)]
   if ((st = zlocal_impl(a, c * d1 * d2, f, d1 * d2 * w2, 1, x1, Hm, x2, 0, stream))) return st;
   // out[(a,ko,lo), g] = x2[(a,ko,lo), (d,f)] R[g, (d,f)]^T
-  return zgemm(SEEMPS_OP_N, SEEMPS_OP_T, a * d1 * d2, g, w2 * f, one, x2, w2 * f, 0, R, w2 * f, 0,
-               zero, out, g, 0, 1, stream);
+  return zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, a * d1 * d2, g, w2 * f, one, x2, w2 * f, R, w2 * f,
+                  zero, out, g, ks, ks_elems, stream);
 }
 
 }  // namespace seemps
@@ -384,7 +523,8 @@ extern "C" int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, c
 
 extern "C" int64_t seemps_two_site_matvec_workspace(int64_t a, int64_t c, int64_t b, int64_t d1,
                                                     int64_t d2, int64_t w2, int64_t f) {
-  return al256(a * c * d1 * d2 * f * 16) + al256(a * d1 * d2 * w2 * f * 16);
+  return al256(a * c * d1 * d2 * f * 16) + al256(a * d1 * d2 * w2 * f * 16) +
+         al256(matvec_ks_elems(a, c, b, d1, d2, w2, f, a) * 16);
 }
 
 extern "C" int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2,
@@ -396,9 +536,12 @@ extern "C" int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d
   SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "two_site_matvec: w*d1*d2 > 64 not supported");
   double2* x1 = (double2*)work;
   double2* x2 = (double2*)((char*)work + al256(a * c * d1 * d2 * f * 16));
+  double2* ks = (double2*)((char*)x2 + al256(a * d1 * d2 * w2 * f * 16));
+  // the split-K size depends on g too; the workspace query assumed g = a
+  const int64_t ks_avail = (work_bytes - ((char*)ks - (char*)work)) / 16;
   return two_site_matvec_impl(a, c, b, d1, d2, w2, f, g, (const double2*)L, (const double2*)Hm,
-                              (const double2*)R, (const double2*)v, (double2*)out, x1, x2,
-                              (hipStream_t)stream);
+                              (const double2*)R, (const double2*)v, (double2*)out, x1, x2, ks,
+                              ks_avail, (hipStream_t)stream);
 }
 
 extern "C" int64_t seemps_eigensolve_workspace(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
@@ -430,6 +573,7 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
   const double2* H_ = (const double2*)Hm;
   const double2* R_ = (const double2*)R;
   const unsigned gb = grid_for(n);
+  const int nbk = (int)reduce_blocks(n);
   int st;
   // V[0] = v0 / ||v0||
   SEEMPS_HIP(hipMemcpyAsync(w.V, v0, n * 16, hipMemcpyDeviceToDevice, st_));
@@ -445,22 +589,21 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
       double2* vj = w.V + j * n;
       double2* wv = w.V + (j + 1) * n;
       if ((st = two_site_matvec_impl(chiL, c, chiL, d1, d2, w2, chiR, chiR, L_, H_, R_, vj, wv,
-                                     w.x1, w.x2, st_)))
+                                     w.x1, w.x2, w.ks,
+                                     matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR),
+                                     st_)))
         return st;
       ++matvecs;
-      // Two classical Gram-Schmidt passes against V[0..j].
-      if ((st = zmdotc_impl(n, j + 1, w.V, n, wv, w.sc->h, w.part, st_))) return st;
-      hipLaunchKernelGGL(maxpy_kernel, dim3(gb), dim3(256), 0, st_, n, (int)(j + 1), w.V, n,
-                         w.sc->h, wv);
-      if ((st = check_launch("maxpy"))) return st;
-      if ((st = zmdotc_impl(n, j + 1, w.V, n, wv, w.sc->h2, w.part, st_))) return st;
-      hipLaunchKernelGGL(maxpy_kernel, dim3(gb), dim3(256), 0, st_, n, (int)(j + 1), w.V, n,
-                         w.sc->h2, wv);
-      if ((st = check_launch("maxpy"))) return st;
-      if ((st = zmdotc_impl(n, 1, wv, n, wv, &w.sc->nrm, w.part, st_))) return st;
-      hipLaunchKernelGGL(lz_record, dim3(1), dim3(64), 0, st_, w.sc, (int)j);
-      hipLaunchKernelGGL(lz_normalize, dim3(gb), dim3(256), 0, st_, n, w.sc, (int)j, wv, wv);
-      if ((st = check_launch("lz_normalize"))) return st;
+      // Two classical Gram-Schmidt passes against V[0..j], then the norm:
+      // four launches (dots | subtract + dots | subtract + norm | normalise).
+      const int kk = (int)(j + 1);
+      hipLaunchKernelGGL(lz_dots, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part);
+      hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part, w.part2,
+                         w.pnorm, w.sc->h, 0);
+      hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part2, w.part,
+                         w.pnorm, w.sc->h2, 1);
+      hipLaunchKernelGGL(lz_finish, dim3(gb), dim3(256), 0, st_, n, nbk, (int)j, w.pnorm, w.sc, wv);
+      if ((st = check_launch("lanczos step"))) return st;
     }
     hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)m);
     if ((st = check_launch("lz_tridiag"))) return st;

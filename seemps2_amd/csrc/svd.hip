@@ -1,38 +1,53 @@
-// FP64 complex SVD (one-sided block Jacobi) and the Strategy truncation rule.
+// FP64 complex SVD (QR-preconditioned one-sided block Jacobi) and the Strategy
+// truncation rule.
 //
-// Algorithm (Hestenes one-sided Jacobi, blocked):
-//   The working matrix W (ncol columns of length len; A itself when m >= n,
-//   A^H otherwise) is split into column blocks of width BW.  A sweep visits every
-//   pair of blocks once (round-robin "circle" schedule, nb/2 disjoint pairs per
-//   step).  For a pair, the 2BW x 2BW Gram matrix G = W_pq^H W_pq is formed,
-//   an inner cyclic Jacobi sweep on G accumulates a unitary Q, and the columns
-//   W_pq <- W_pq Q and V_pq <- V_pq Q are rotated.  Sweeps stop when every
-//   normalised off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in a sweep is below
-//   8 sqrt(len) eps.  Then sigma_k = ||w_k||, U = W / sigma, V accumulated.
+// Pipeline for an m x n matrix A (B = A if m >= n, else B = A^H; B is
+// len x ncol with len >= ncol):
+//   1. B is copied column-major (column c contiguous).
+//   2. If ncol > 16: blocked Householder QR, B = Q R (qr.hip).  The Jacobi then
+//      works on X = R^H (ncol x ncol); Jacobi on R^H converges in a handful of
+//      sweeps even for graded spectra (Drmac-Veselic preconditioning).
+//      Otherwise the Jacobi works on B directly.
+//   3. One-sided block Jacobi on the working matrix W (columns of X or B):
+//      column blocks of width BW, every pair of blocks visited once per sweep
+//      in a round-robin (circle) schedule; per pair the 2BW x 2BW Gram matrix is
+//      formed (partial sums over row chunks in different workgroups), an inner
+//      Hermitian Jacobi sweep accumulates Q, and W_pq <- W_pq Q, V_pq <- V_pq Q.
+//      Stops when every normalised off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in
+//      a sweep is below 8 sqrt(len) eps (relative orthogonality).
+//   4. sigma_k = ||w_k||, sorted descending; the W side gives the normalised
+//      singular vectors of the working matrix, V (times Q in the QR path) the
+//      other side.  Exactly-zero singular values get an orthonormal completion.
 //
-// Parallel mapping: one cooperative persistent kernel for all sweeps.  Each step
-// has nb/2 pairs x R row chunks work items; phase A writes per-chunk partial
-// Grams, a grid barrier, phase B reduces them and runs the inner Jacobi sweep
-// (redundantly in each chunk's workgroup, bitwise identical), phase C rotates
-// its chunk of rows, a grid barrier.  Barrier waits are bounded: a workgroup
-// that spins too long raises an abort flag that every other waiter observes, so
-// all waves always exit.
+// The Jacobi runs as one cooperative persistent kernel: two bounded grid
+// barriers per block step.  A workgroup that waits too long raises an abort
+// flag that every waiter observes, so every wave always exits.
 //
 // Replaces _destructive_svd (src/seemps/cython/svd.pxi:17-83) and the
 // truncation rules of src/seemps/cython/truncation.pxi:222-334.
 #include <cmath>
+#include <cstdlib>
 
 #include "common.h"
 
 namespace seemps {
+
+int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
+              double2* tau, double2* Z, hipStream_t stream);
+int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,
+               const double2* Tall, double2* Z, hipStream_t stream);
+
 namespace {
 
 constexpr int BW = 16;          // block width (columns)
 constexpr int PW = 2 * BW;      // pair width
+constexpr int GP = PW + 1;      // padded LDS row stride of G and Q (complex)
 constexpr int RT = 32;          // rows per LDS tile
 constexpr int RTP = RT + 1;     // padded tile row length (complex)
 constexpr int RMAX = 16;        // max row chunks per pair
-constexpr int MAX_SWEEPS = 40;
+constexpr int QRNB = 16;        // QR panel width
+constexpr int QR_MIN_COLS = 17; // use the QR preconditioner from this many columns
+constexpr int MAX_SWEEPS = 60;
 constexpr unsigned long long SPIN_LIMIT = 4000000ull;
 
 struct Ctrl {
@@ -40,17 +55,28 @@ struct Ctrl {
   unsigned int abort;         // set when a barrier wait timed out
   unsigned long long off[2];  // per-sweep max normalised off-diagonal (as bits)
   int info[4];                // sweeps, converged, aborted, unused
+  long long prof[8];          // wall-clock ticks of workgroup 0 per phase
+  double frob2;               // ||W||_F^2 (invariant under the rotations)
+  double tiny2;               // squared norm below which a column is negligible
 };
 
 struct SvdShape {
   int64_t m, n;         // input shape
-  int64_t len, ncol;    // working matrix: ncol columns of length len
+  int64_t lenA, ncol;   // working matrix B: ncol columns of length lenA
+  int64_t len;          // Jacobi column length (ncol with QR, lenA without)
   int64_t nb, ncolp;    // blocks (even), padded column count
   int64_t P, R;         // pairs per step, row chunks
-  int transposed;       // 1 if W = A^H
+  int transposed;       // 1 if B = A^H
+  int use_qr;
 };
 
 struct SvdWork {
+  double2* Acm;   // ncol x lenA (QR in place)
+  double2* Vall;  // ncol x lenA (QR vectors)
+  double2* Tall;  // npanels x QRNB x QRNB
+  double2* tau;   // ncol
+  double2* Z;     // 2 x QRNB x max(ncol, ncolp)
+  double2* Y;     // ncol x lenA (Q V)
   double2* W;     // ncolp x len
   double2* V;     // ncolp x ncolp
   double2* part;  // P x RMAX x PW x PW
@@ -66,8 +92,10 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
   s.m = m;
   s.n = n;
   s.transposed = (m < n);
-  s.len = s.transposed ? n : m;
+  s.lenA = s.transposed ? n : m;
   s.ncol = s.transposed ? m : n;
+  s.use_qr = (s.ncol >= QR_MIN_COLS);
+  s.len = s.use_qr ? s.ncol : s.lenA;
   int64_t nb = ceil_div(s.ncol, BW);
   if (nb < 2) nb = 2;
   nb += (nb & 1);
@@ -83,41 +111,47 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
   return s;
 }
 
+#define SEEMPS_SVD_REGIONS(X)                                         \
+  X(Acm, s.use_qr ? s.ncol * s.lenA : 0, double2)                     \
+  X(Vall, s.use_qr ? s.ncol * s.lenA : 0, double2)                    \
+  X(Tall, s.use_qr ? ceil_div(s.ncol, QRNB) * QRNB * QRNB : 0, double2) \
+  X(tau, s.use_qr ? s.ncol : 0, double2)                              \
+  X(Z, s.use_qr ? 2 * QRNB * (s.ncolp > s.lenA ? s.ncolp : s.lenA) : 0, double2) \
+  X(Y, s.use_qr ? s.ncol * s.lenA : 0, double2)                       \
+  X(W, s.ncolp * s.len, double2)                                      \
+  X(V, s.ncolp * s.ncolp, double2)                                    \
+  X(part, s.P * RMAX * PW * PW, double2)                              \
+  X(sigma, s.ncolp, double)                                           \
+  X(perm, s.ncolp, int)                                               \
+  X(ctrl, 1, Ctrl)
+
 int64_t svd_bytes(const SvdShape& s) {
   int64_t b = 0;
-  b += align256(s.ncolp * s.len * 16);
-  b += align256(s.ncolp * s.ncolp * 16);
-  b += align256(s.P * RMAX * PW * PW * 16);
-  b += align256(s.ncolp * 8);
-  b += align256(s.ncolp * 4);
-  b += align256(sizeof(Ctrl));
+#define X(name, count, type) b += align256((int64_t)(count) * (int64_t)sizeof(type));
+  SEEMPS_SVD_REGIONS(X)
+#undef X
   return b;
 }
 
 SvdWork svd_carve(const SvdShape& s, void* work) {
   char* p = (char*)work;
   SvdWork w;
-  w.W = (double2*)p;
-  p += align256(s.ncolp * s.len * 16);
-  w.V = (double2*)p;
-  p += align256(s.ncolp * s.ncolp * 16);
-  w.part = (double2*)p;
-  p += align256(s.P * RMAX * PW * PW * 16);
-  w.sigma = (double*)p;
-  p += align256(s.ncolp * 8);
-  w.perm = (int*)p;
-  p += align256(s.ncolp * 4);
-  w.ctrl = (Ctrl*)p;
+#define X(name, count, type)                                   \
+  w.name = (type*)p;                                           \
+  p += align256((int64_t)(count) * (int64_t)sizeof(type));
+  SEEMPS_SVD_REGIONS(X)
+#undef X
   return w;
 }
 
-// W[c][r] = A[r][c] (m >= n) or conj(A[c][r]) (m < n); V = identity; ctrl = 0.
-__global__ void svd_prepare(SvdShape s, const double2* __restrict__ A, int64_t lda, SvdWork w) {
-  const int64_t totalW = s.ncolp * s.len;
-  const int64_t totalV = s.ncolp * s.ncolp;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < totalW; e += stride) {
-    const int64_t c = e / s.len, r = e % s.len;
+// B column-major: dst[c * lenA + r] = A[r][c] (m >= n) or conj(A[c][r]) (m < n),
+// zero for padded columns c >= ncol (dst has ncols_dst columns).
+__global__ void load_colmajor(SvdShape s, const double2* __restrict__ A, int64_t lda,
+                              double2* __restrict__ dst, int64_t ncols_dst) {
+  const int64_t total = ncols_dst * s.lenA;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / s.lenA, r = e % s.lenA;
     double2 v = make_double2(0.0, 0.0);
     if (c < s.ncol) {
       if (!s.transposed) {
@@ -127,9 +161,31 @@ __global__ void svd_prepare(SvdShape s, const double2* __restrict__ A, int64_t l
         v.y = -v.y;
       }
     }
-    w.W[e] = v;
+    dst[e] = v;
   }
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < totalV; e += stride) {
+}
+
+// W[c][k] = conj(R[c][k]) = conj(Acm[k][c]) for k >= c (columns of X = R^H).
+__global__ void load_w_from_r(SvdShape s, const double2* __restrict__ Acm,
+                              double2* __restrict__ W) {
+  const int64_t total = s.ncolp * s.len;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / s.len, k = e % s.len;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < s.ncol && k >= c) {
+      v = Acm[k * s.lenA + c];
+      v.y = -v.y;
+    }
+    W[e] = v;
+  }
+}
+
+// V = identity on the real columns; control block reset.
+__global__ void init_v_ctrl(SvdShape s, SvdWork w) {
+  const int64_t totalV = s.ncolp * s.ncolp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < totalV;
+       e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = e / s.ncolp, r = e % s.ncolp;
     w.V[e] = make_double2((c == r && c < s.ncol) ? 1.0 : 0.0, 0.0);
   }
@@ -138,7 +194,37 @@ __global__ void svd_prepare(SvdShape s, const double2* __restrict__ A, int64_t l
     w.ctrl->abort = 0;
     w.ctrl->off[0] = 0;
     w.ctrl->off[1] = 0;
-    w.ctrl->info[0] = w.ctrl->info[1] = w.ctrl->info[2] = w.ctrl->info[3] = 0;
+    for (int i = 0; i < 4; ++i) w.ctrl->info[i] = 0;
+    for (int i = 0; i < 8; ++i) w.ctrl->prof[i] = 0;
+  }
+}
+
+// ctrl->frob2 = ||W||_F^2 (fixed summation order), ctrl->tiny2 = (len eps)^2 frob2.
+// One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void svd_frob(SvdShape s, SvdWork w) {
+  __shared__ double red[16];
+  const int64_t total = s.ncolp * s.len;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < total; e += blockDim.x) acc += cabs2(w.W[e]);
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += red[q];
+    const double le = (double)s.len * 2.220446049250313e-16;
+    w.ctrl->frob2 = t;
+    w.ctrl->tiny2 = le * le * t;
+  }
+}
+
+// Y[c][r] = V[c][r] for r < ncol, 0 below (input of Q V).
+__global__ void load_y(SvdShape s, SvdWork w) {
+  const int64_t total = s.ncol * s.lenA;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / s.lenA, r = e % s.lenA;
+    w.Y[e] = (r < s.ncol) ? w.V[c * s.ncolp + r] : make_double2(0.0, 0.0);
   }
 }
 
@@ -150,7 +236,7 @@ __device__ bool grid_barrier(Ctrl* ctrl, unsigned int target, int* s_flag) {
     atomicAdd(&ctrl->barrier, 1u);
     unsigned long long spins = 0;
     int ok = 1;
-    while (__hip_atomic_load(&ctrl->barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+    while (__hip_atomic_load(&ctrl->barrier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
            target) {
       if (__hip_atomic_load(&ctrl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         ok = 0;
@@ -161,7 +247,7 @@ __device__ bool grid_barrier(Ctrl* ctrl, unsigned int target, int* s_flag) {
         ok = 0;
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
     __threadfence();
     *s_flag = ok;
@@ -188,13 +274,14 @@ __device__ __forceinline__ int64_t pair_col(int64_t bi, int64_t bj, int t) {
   return (t < BW) ? bi * BW + t : bj * BW + (t - BW);
 }
 
-// Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS),
-// accumulating Q (LDS, on entry identity).  256 threads.
-__device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, int sweeps) {
+// Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS, row
+// stride GP), accumulating Q (LDS, on entry identity).  256 threads.  Pairs
+// with a negligible column (squared norm <= tiny2) are not rotated.
+__device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, double tiny2,
+                             int sweeps) {
   const int tid = threadIdx.x;
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int k = 0; k < PW - 1; ++k) {
-      // Pair t of round k over PW indices (circle schedule).
       if (tid < PW / 2) {
         const int t = tid;
         int a, b;
@@ -206,12 +293,12 @@ __device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, i
           a = idx(t);
           b = idx(PW - 1 - t);
         }
-        const double gaa = G[a * PW + a].x, gbb = G[b * PW + b].x;
-        const double2 c = G[a * PW + b];
+        const double gaa = G[a * GP + a].x, gbb = G[b * GP + b].x;
+        const double2 c = G[a * GP + b];
         const double ac = sqrt(cabs2(c));
         double cs = 1.0, sn = 0.0;
         double2 ph = make_double2(1.0, 0.0);
-        if (ac > 0.0 && gaa > 0.0 && gbb > 0.0 && ac > tol * sqrt(gaa) * sqrt(gbb)) {
+        if (ac > 0.0 && gaa > tiny2 && gbb > tiny2 && ac > tol * sqrt(gaa) * sqrt(gbb)) {
           ph = make_double2(c.x / ac, c.y / ac);
           const double tau = (gbb - gaa) / (2.0 * ac);
           double t_;
@@ -231,19 +318,18 @@ __device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, i
       for (int e = tid; e < 2 * PW * (PW / 2); e += blockDim.x) {
         const int which = e / (PW * (PW / 2));  // 0: G, 1: Q
         const int rem = e % (PW * (PW / 2));
-        const int t = rem / PW, x = rem % PW;
+        const int x = rem / (PW / 2), t = rem % (PW / 2);  // lanes of a row share x
         const double cs = rot[3 * t].x, sn = rot[3 * t].y;
         if (sn == 0.0) continue;
         const double2 ph = rot[3 * t + 1];
         const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
         double2* M = which ? Q : G;
-        const double2 xa = M[x * PW + a], xb = M[x * PW + b];
+        const double2 xa = M[x * GP + a], xb = M[x * GP + b];
         const double2 phc = make_double2(ph.x, -ph.y);
-        const double2 na = make_double2(cs * xa.x, cs * xa.y);
         const double2 tb = cmul(phc, xb);
         const double2 ta = cmul(ph, xa);
-        M[x * PW + a] = make_double2(na.x - sn * tb.x, na.y - sn * tb.y);
-        M[x * PW + b] = make_double2(sn * ta.x + cs * xb.x, sn * ta.y + cs * xb.y);
+        M[x * GP + a] = make_double2(cs * xa.x - sn * tb.x, cs * xa.y - sn * tb.y);
+        M[x * GP + b] = make_double2(sn * ta.x + cs * xb.x, sn * ta.y + cs * xb.y);
       }
       __syncthreads();
       // Row pass: G <- J^H G.
@@ -253,21 +339,21 @@ __device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, i
         if (sn == 0.0) continue;
         const double2 ph = rot[3 * t + 1];
         const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
-        const double2 ay = G[a * PW + y], by = G[b * PW + y];
+        const double2 ay = G[a * GP + y], by = G[b * GP + y];
         const double2 phc = make_double2(ph.x, -ph.y);
         const double2 t1 = cmul(ph, by), t2 = cmul(phc, ay);
-        G[a * PW + y] = make_double2(cs * ay.x - sn * t1.x, cs * ay.y - sn * t1.y);
-        G[b * PW + y] = make_double2(sn * t2.x + cs * by.x, sn * t2.y + cs * by.y);
+        G[a * GP + y] = make_double2(cs * ay.x - sn * t1.x, cs * ay.y - sn * t1.y);
+        G[b * GP + y] = make_double2(sn * t2.x + cs * by.x, sn * t2.y + cs * by.y);
       }
       __syncthreads();
       if (tid < PW / 2) {
         const int t = tid;
         if (rot[3 * t].y != 0.0) {
           const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
-          G[a * PW + b] = make_double2(0.0, 0.0);
-          G[b * PW + a] = make_double2(0.0, 0.0);
-          G[a * PW + a].y = 0.0;
-          G[b * PW + b].y = 0.0;
+          G[a * GP + b] = make_double2(0.0, 0.0);
+          G[b * GP + a] = make_double2(0.0, 0.0);
+          G[a * GP + a].y = 0.0;
+          G[b * GP + b].y = 0.0;
         }
       }
       __syncthreads();
@@ -292,7 +378,7 @@ __device__ void rotate_rows(double2* __restrict__ M, int64_t ld, int64_t bi, int
       if (rr < nr) {
         double2 acc = make_double2(0.0, 0.0);
 #pragma unroll 8
-        for (int k = 0; k < PW; ++k) acc = cfma(tile[k * RTP + rr], Q[k * PW + j], acc);
+        for (int k = 0; k < PW; ++k) acc = cfma(tile[k * RTP + rr], Q[k * GP + j], acc);
         M[pair_col(bi, bj, j) * ld + rb + rr] = acc;
       }
     }
@@ -300,26 +386,33 @@ __device__ void rotate_rows(double2* __restrict__ M, int64_t ld, int64_t bi, int
   }
 }
 
+__device__ __forceinline__ long long now_ticks() { return (long long)wall_clock64(); }
+
 __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double tol,
                                                   int max_sweeps) {
   __shared__ double2 tile[PW * RTP];
-  __shared__ double2 G[PW * PW];
-  __shared__ double2 Q[PW * PW];
+  __shared__ double2 G[PW * GP];
+  __shared__ double2 Q[PW * GP];
   __shared__ double2 rot[3 * (PW / 2)];
   __shared__ int s_flag;
-  __shared__ double s_off[256];
+  __shared__ double s_off[8];
 
   const int tid = threadIdx.x;
+  const bool prof = (blockIdx.x == 0 && tid == 0);
+  long long pt[6] = {0, 0, 0, 0, 0, 0};
+  long long t0 = 0;
   const int64_t items = s.P * s.R;
   const int64_t rc = ceil_div(s.len, s.R);      // rows of W per chunk
   const int64_t rcv = ceil_div(s.ncolp, s.R);   // rows of V per chunk
   const int inner_sweeps = (s.nb == 2) ? 3 : 1;
+  const double tiny2 = w.ctrl->tiny2;
   unsigned int barrier_no = 0;
   const unsigned int G_ = gridDim.x;
   int sweep = 0;
   int converged = 0;
   for (; sweep < max_sweeps; ++sweep) {
     for (int64_t k = 0; k < s.nb - 1; ++k) {
+      if (prof) t0 = now_ticks();
       // Phase A: partial Gram matrices.
       for (int64_t it = blockIdx.x; it < items; it += G_) {
         const int64_t p = it / s.R, r = it % s.R;
@@ -350,7 +443,17 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
 #pragma unroll
         for (int u = 0; u < 4; ++u) dst[tid + u * 256] = acc[u];
       }
+      if (prof) {
+        const long long t1 = now_ticks();
+        pt[0] += t1 - t0;
+        t0 = t1;
+      }
       if (!grid_barrier(w.ctrl, (++barrier_no) * G_, &s_flag)) goto aborted;
+      if (prof) {
+        const long long t1 = now_ticks();
+        pt[1] += t1 - t0;
+        t0 = t1;
+      }
       // Every workgroup has read the previous sweep's flag (slot (sweep+1)&1)
       // before arriving here: it can now be cleared for the next sweep.
       if (k == 0 && blockIdx.x == 0 && tid == 0) w.ctrl->off[(sweep + 1) & 1] = 0ull;
@@ -362,35 +465,62 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
         pair_blocks(s.nb, k, p, &bi, &bj);
         const double2* src = w.part + p * RMAX * PW * PW;
         for (int e = tid; e < PW * PW; e += blockDim.x) {
-          double2 g = make_double2(0.0, 0.0);
-          for (int64_t q = 0; q < s.R; ++q) g = cadd(g, src[q * PW * PW + e]);
-          G[e] = g;
-          Q[e] = make_double2((e / PW == e % PW) ? 1.0 : 0.0, 0.0);
+          double2 gq[RMAX];
+#pragma unroll
+          for (int q = 0; q < RMAX; ++q)
+            gq[q] = (q < s.R) ? src[q * PW * PW + e] : make_double2(0.0, 0.0);
+          double2 g = gq[0];
+#pragma unroll
+          for (int q = 1; q < RMAX; ++q) g = cadd(g, gq[q]);
+          const int i = e / PW, j = e % PW;
+          G[i * GP + j] = g;
+          Q[i * GP + j] = make_double2((i == j) ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
         if (r == 0) {
+          // max normalised off-diagonal over non-negligible column pairs
           double mx = 0.0;
           for (int e = tid; e < PW * PW; e += blockDim.x) {
             const int i = e / PW, j = e % PW;
             if (i < j) {
-              const double gi = G[i * PW + i].x, gj = G[j * PW + j].x;
-              if (gi > 0.0 && gj > 0.0) mx = fmax(mx, sqrt(cabs2(G[e])) / sqrt(gi) / sqrt(gj));
+              const double gi = G[i * GP + i].x, gj = G[j * GP + j].x;
+              if (gi > tiny2 && gj > tiny2) {
+                const double2 gij = G[i * GP + j];
+                mx = fmax(mx, sqrt(cabs2(gij) / gi / gj));
+              }
             }
           }
-          s_off[tid] = mx;
+          for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+          if ((tid & 63) == 0) s_off[tid >> 6] = mx;
           __syncthreads();
           if (tid == 0) {
             double m2 = 0.0;
-            for (int q = 0; q < (int)blockDim.x; ++q) m2 = fmax(m2, s_off[q]);
+            for (int q = 0; q < (int)(blockDim.x >> 6); ++q) m2 = fmax(m2, s_off[q]);
             atomicMax(&w.ctrl->off[sweep & 1], (unsigned long long)__double_as_longlong(m2));
           }
           __syncthreads();
         }
-        inner_jacobi(G, Q, rot, tol, inner_sweeps);
+        if (prof) {
+          const long long t1 = now_ticks();
+          pt[2] += t1 - t0;
+          t0 = t1;
+        }
+        inner_jacobi(G, Q, rot, tol, tiny2, inner_sweeps);
+        if (prof) {
+          const long long t1 = now_ticks();
+          pt[3] += t1 - t0;
+          t0 = t1;
+        }
         rotate_rows(w.W, s.len, bi, bj, r * rc, min(s.len, r * rc + rc), Q, tile);
         rotate_rows(w.V, s.ncolp, bi, bj, r * rcv, min(s.ncolp, r * rcv + rcv), Q, tile);
+        if (prof) {
+          const long long t1 = now_ticks();
+          pt[4] += t1 - t0;
+          t0 = t1;
+        }
       }
       if (!grid_barrier(w.ctrl, (++barrier_no) * G_, &s_flag)) goto aborted;
+      if (prof) pt[5] += now_ticks() - t0;
     }
     {
       const unsigned long long bits =
@@ -402,21 +532,22 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
       }
     }
   }
-  if (blockIdx.x == 0 && tid == 0) {
+  if (prof) {
     w.ctrl->info[0] = sweep;
     w.ctrl->info[1] = converged;
+    for (int i = 0; i < 6; ++i) w.ctrl->prof[i] = pt[i];
   }
   return;
 aborted:
   if (blockIdx.x == 0 && tid == 0) w.ctrl->info[2] = 1;
 }
 
-// sigma_c = ||w_c|| and descending ranks (ties by index).
+// sigma_c = ||w_c||.
 __global__ void svd_sigma(SvdShape s, SvdWork w) {
+  __shared__ double red[256];
   for (int64_t c = blockIdx.x; c < s.ncolp; c += gridDim.x) {
     double acc = 0.0;
     for (int64_t r = threadIdx.x; r < s.len; r += blockDim.x) acc += cabs2(w.W[c * s.len + r]);
-    __shared__ double red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -428,6 +559,7 @@ __global__ void svd_sigma(SvdShape s, SvdWork w) {
   }
 }
 
+// perm[rank] = c with descending sigma, ties by index.
 __global__ void svd_rank(SvdShape s, SvdWork w) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncolp;
        c += (int64_t)gridDim.x * blockDim.x) {
@@ -441,66 +573,79 @@ __global__ void svd_rank(SvdShape s, SvdWork w) {
   }
 }
 
-// Write U (m x r), S (r), VH (r x n), r = ncol.
+// Output.  W side: normalised columns of the working matrix (length len);
+// V side: V (no QR; stride ncolp, length ncol) or Y = Q V (QR; stride lenA,
+// length lenA).  wside_u: the W side is U (columns) rather than VH (conj rows).
 __global__ void svd_output(SvdShape s, SvdWork w, double2* __restrict__ U, double* __restrict__ S,
                            double2* __restrict__ VH) {
   const int64_t r = s.ncol;
+  const int wside_u = (s.use_qr == s.transposed);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (int64_t k = t0; k < r; k += stride) S[k] = w.sigma[w.perm[k]];
-  // Side built from W (normalised columns): U when m >= n, VH when m < n.
+  const double tiny2 = w.ctrl->tiny2;
   for (int64_t e = t0; e < r * s.len; e += stride) {
     const int64_t k = e / s.len, x = e % s.len;
     const int64_t c = w.perm[k];
     const double sg = w.sigma[c];
     double2 v = w.W[c * s.len + x];
-    v = (sg > 0.0) ? make_double2(v.x / sg, v.y / sg) : make_double2(0.0, 0.0);
-    if (!s.transposed)
-      U[x * r + k] = v;  // U[x][k], m x r
+    // negligible columns get an orthonormal completion afterwards
+    v = (sg > 0.0 && sg * sg > tiny2) ? make_double2(v.x / sg, v.y / sg) : make_double2(0.0, 0.0);
+    if (wside_u)
+      U[x * r + k] = v;
     else
-      VH[k * s.n + x] = make_double2(v.x, -v.y);  // VH[k][x] = conj(W_k[x])
+      VH[k * s.n + x] = make_double2(v.x, -v.y);
   }
-  // Side built from V: VH when m >= n, U when m < n.
-  for (int64_t e = t0; e < r * s.ncol; e += stride) {
-    const int64_t k = e / s.ncol, x = e % s.ncol;
-    const double2 v = w.V[(int64_t)w.perm[k] * s.ncolp + x];
-    if (!s.transposed)
+  const double2* src = s.use_qr ? w.Y : w.V;
+  const int64_t ld = s.use_qr ? s.lenA : s.ncolp;
+  const int64_t vlen = s.use_qr ? s.lenA : s.ncol;
+  for (int64_t e = t0; e < r * vlen; e += stride) {
+    const int64_t k = e / vlen, x = e % vlen;
+    const double2 v = src[(int64_t)w.perm[k] * ld + x];
+    if (wside_u)
       VH[k * s.n + x] = make_double2(v.x, -v.y);
     else
       U[x * r + k] = v;
   }
 }
 
-// Complete the W-side singular vectors of exactly-zero singular values to an
-// orthonormal set (classical Gram-Schmidt twice against unit candidates e_j).
-// One workgroup; only launched when such columns exist.
+// Complete the W-side singular vectors of negligible singular values
+// (sigma^2 <= tiny2: at the rounding level of the factorisation, where the
+// directions are not determined) to an orthonormal set, by classical
+// Gram-Schmidt twice against unit candidates.  The singular values themselves
+// are kept.  One workgroup; loops over nothing in the common case.
 __global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, double2* __restrict__ U,
                                                     double2* __restrict__ VH) {
   const int64_t r = s.ncol, len = s.len;
-  // Vector k of the W side: element x at X(k, x).
-  auto addr = [&](int64_t k, int64_t x) -> double2* {
-    return !s.transposed ? &U[x * r + k] : &VH[k * s.n + x];
+  const int wside_u = (s.use_qr == s.transposed);
+  // element x of W-side vector k, as a plain (unconjugated) vector component
+  auto load = [&](int64_t k, int64_t x) -> double2 {
+    if (wside_u) return U[x * r + k];
+    const double2 v = VH[k * s.n + x];
+    return make_double2(v.x, -v.y);
+  };
+  auto store = [&](int64_t k, int64_t x, double2 v) {
+    if (wside_u)
+      U[x * r + k] = v;
+    else
+      VH[k * s.n + x] = make_double2(v.x, -v.y);
   };
   __shared__ double2 red[256];
   __shared__ double2 coef;
   int64_t cand = 0;
+  const double tiny2 = w.ctrl->tiny2;
   for (int64_t k = 0; k < r; ++k) {
-    if (w.sigma[w.perm[k]] > 0.0) continue;
+    const double sg = w.sigma[w.perm[k]];
+    if (sg > 0.0 && sg * sg > tiny2) continue;
     for (; cand < len; ++cand) {
       for (int64_t x = threadIdx.x; x < len; x += blockDim.x)
-        *addr(k, x) = make_double2(x == cand ? 1.0 : 0.0, 0.0);
+        store(k, x, make_double2(x == cand ? 1.0 : 0.0, 0.0));
       __syncthreads();
       for (int pass = 0; pass < 2; ++pass) {
         for (int64_t j = 0; j < k; ++j) {
           double2 acc = make_double2(0.0, 0.0);
-          for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
-            double2 a = *addr(j, x), b = *addr(k, x);
-            if (s.transposed) {  // rows of VH are conjugated vectors
-              a.y = -a.y;
-              b.y = -b.y;
-            }
-            acc = cfmac(a, b, acc);
-          }
+          for (int64_t x = threadIdx.x; x < len; x += blockDim.x)
+            acc = cfmac(load(j, x), load(k, x), acc);
           red[threadIdx.x] = acc;
           __syncthreads();
           if (threadIdx.x == 0) {
@@ -509,22 +654,17 @@ __global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, doubl
             coef = t;
           }
           __syncthreads();
+          const double2 cf = coef;
           for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
-            double2 a = *addr(j, x), b = *addr(k, x);
-            if (s.transposed) {
-              a.y = -a.y;
-              b.y = -b.y;
-            }
-            const double2 pa = cmul(coef, a);
-            b = make_double2(b.x - pa.x, b.y - pa.y);
-            if (s.transposed) b.y = -b.y;
-            *addr(k, x) = b;
+            const double2 pa = cmul(cf, load(j, x));
+            const double2 b = load(k, x);
+            store(k, x, make_double2(b.x - pa.x, b.y - pa.y));
           }
           __syncthreads();
         }
       }
       double acc = 0.0;
-      for (int64_t x = threadIdx.x; x < len; x += blockDim.x) acc += cabs2(*addr(k, x));
+      for (int64_t x = threadIdx.x; x < len; x += blockDim.x) acc += cabs2(load(k, x));
       red[threadIdx.x] = make_double2(acc, 0.0);
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -534,24 +674,26 @@ __global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, doubl
       }
       __syncthreads();
       const double nrm = coef.x;
+      __syncthreads();
       if (nrm > 0.5) {
         for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
-          double2 b = *addr(k, x);
-          *addr(k, x) = make_double2(b.x / nrm, b.y / nrm);
+          const double2 b = load(k, x);
+          store(k, x, make_double2(b.x / nrm, b.y / nrm));
         }
         __syncthreads();
         ++cand;
         break;
       }
-      __syncthreads();
     }
   }
 }
 
 // Truncation rule: one lane runs the reference's sequential loops over an LDS
-// copy of s (summation order identical to truncation.pxi:251-327).
+// copy of s (summation order identical to truncation.pxi:251-327, products and
+// sums rounded separately as in the reference's compiled loops).
 __global__ void truncate_kernel(int64_t N, const double* __restrict__ s, int method, double tol,
                                 int64_t max_bond, int64_t* D_out, double* err_out) {
+#pragma clang fp contract(off)
   extern __shared__ double ss[];
   for (int64_t i = threadIdx.x; i < N; i += blockDim.x) ss[i] = s[i];
   __syncthreads();
@@ -581,8 +723,7 @@ __global__ void truncate_kernel(int64_t N, const double* __restrict__ s, int met
     }
     D = N - i_stop;
     if (D > max_bond) D = max_bond;
-    // err = errors[N - D]
-    double e = 0.0;
+    double e = 0.0;  // errors[N - D]
     for (int64_t i = 0; i < N - D; ++i) {
       const double x = ss[N - 1 - i];
       e += x * x;
@@ -627,10 +768,18 @@ int cooperative_grid(int64_t items, int* grid) {
   return SEEMPS_OK;
 }
 
+unsigned blocks_for(int64_t n) {
+  int64_t b = ceil_div(n, 256);
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
 }  // namespace
 
 int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
-                double2* VH, void* work, int64_t work_bytes, hipStream_t stream, int* info_dev_off) {
+                double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
+                int* info_dev_off) {
   DeviceInfo di;
   int st = device_info(&di);
   if (st) return st;
@@ -638,10 +787,24 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   SEEMPS_CHECK_ARG(work_bytes >= svd_bytes(s), "svd: workspace too small (%lld < %lld)",
                    (long long)work_bytes, (long long)svd_bytes(s));
   SvdWork w = svd_carve(s, work);
-  int64_t blocks = ceil_div(s.ncolp * (s.len > s.ncolp ? s.len : s.ncolp), 256);
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(svd_prepare, dim3((unsigned)blocks), dim3(256), 0, stream, s, A, lda, w);
-  if ((st = check_launch("svd_prepare"))) return st;
+  if (s.use_qr) {
+    hipLaunchKernelGGL(load_colmajor, dim3(blocks_for(s.ncol * s.lenA)), dim3(256), 0, stream, s,
+                       A, lda, w.Acm, s.ncol);
+    if ((st = check_launch("load_colmajor"))) return st;
+    if ((st = qr_factor(w.Acm, s.lenA, s.ncol, QRNB, w.Vall, w.Tall, w.tau, w.Z, stream)))
+      return st;
+    hipLaunchKernelGGL(load_w_from_r, dim3(blocks_for(s.ncolp * s.len)), dim3(256), 0, stream, s,
+                       w.Acm, w.W);
+    if ((st = check_launch("load_w_from_r"))) return st;
+  } else {
+    hipLaunchKernelGGL(load_colmajor, dim3(blocks_for(s.ncolp * s.lenA)), dim3(256), 0, stream,
+                       s, A, lda, w.W, s.ncolp);
+    if ((st = check_launch("load_colmajor"))) return st;
+  }
+  hipLaunchKernelGGL(init_v_ctrl, dim3(blocks_for(s.ncolp * s.ncolp)), dim3(256), 0, stream, s, w);
+  if ((st = check_launch("init_v_ctrl"))) return st;
+  hipLaunchKernelGGL(svd_frob, dim3(1), dim3(1024), 0, stream, s, w);
+  if ((st = check_launch("svd_frob"))) return st;
   int grid = 1;
   if ((st = cooperative_grid(s.P * s.R, &grid))) return st;
   const double tol = 8.0 * sqrt((double)s.len) * 2.220446049250313e-16;
@@ -652,15 +815,20 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   hipLaunchKernelGGL(svd_sigma, dim3((unsigned)(s.ncolp < 1024 ? s.ncolp : 1024)), dim3(256), 0,
                      stream, s, w);
   if ((st = check_launch("svd_sigma"))) return st;
-  hipLaunchKernelGGL(svd_rank, dim3((unsigned)ceil_div(s.ncolp, 256)), dim3(256), 0, stream, s, w);
+  hipLaunchKernelGGL(svd_rank, dim3(blocks_for(s.ncolp)), dim3(256), 0, stream, s, w);
   if ((st = check_launch("svd_rank"))) return st;
-  int64_t ob = ceil_div(s.ncol * (s.len > s.ncol ? s.len : s.ncol), 256);
-  if (ob > 2048) ob = 2048;
-  hipLaunchKernelGGL(svd_output, dim3((unsigned)ob), dim3(256), 0, stream, s, w, U, S, VH);
+  if (s.use_qr) {
+    hipLaunchKernelGGL(load_y, dim3(blocks_for(s.ncol * s.lenA)), dim3(256), 0, stream, s, w);
+    if ((st = check_launch("load_y"))) return st;
+    if ((st = qr_apply_q(w.Y, s.lenA, s.ncol, s.ncol, QRNB, w.Vall, w.Tall, w.Z, stream)))
+      return st;
+  }
+  const int64_t big = s.ncol * (s.len > s.lenA ? s.len : s.lenA);
+  hipLaunchKernelGGL(svd_output, dim3(blocks_for(big)), dim3(256), 0, stream, s, w, U, S, VH);
   if ((st = check_launch("svd_output"))) return st;
   hipLaunchKernelGGL(svd_complete, dim3(1), dim3(256), 0, stream, s, w, U, VH);
   if ((st = check_launch("svd_complete"))) return st;
-  if (info_dev_off) *info_dev_off = (int)((char*)&w.ctrl->info[0] - (char*)work);
+  if (info_dev_off) *info_dev_off = (int)((char*)w.ctrl - (char*)work);
   return SEEMPS_OK;
 }
 
@@ -681,29 +849,46 @@ int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t
   return check_launch("truncate");
 }
 
+// Read the control block after the stream has been synchronised by the caller's copy.
+int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream) {
+  struct {
+    unsigned int barrier, abort;
+    unsigned long long off[2];
+    int info[4];
+    long long prof[8];
+  } h;
+  SEEMPS_HIP(hipMemcpyAsync(&h, (char*)work + off, sizeof(h), hipMemcpyDeviceToHost, stream));
+  SEEMPS_HIP(hipStreamSynchronize(stream));
+  if (info) {
+    info[0] = h.info[0];
+    info[1] = h.info[1];
+  }
+  if (getenv("SEEMPS_SVD_PROFILE")) {
+    int dev = 0, rate = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev);  // kHz
+    const double us = rate > 0 ? 1e3 / rate : 0.0;
+    fprintf(stderr,
+            "[svd] sweeps=%d converged=%d  us: gramA=%.1f bar1=%.1f reduce=%.1f inner=%.1f "
+            "rotate=%.1f bar2=%.1f\n",
+            h.info[0], h.info[1], h.prof[0] * us, h.prof[1] * us, h.prof[2] * us, h.prof[3] * us,
+            h.prof[4] * us, h.prof[5] * us);
+  }
+  if (h.info[2]) {
+    set_error("svd: grid barrier timed out (cooperative residency lost)");
+    return SEEMPS_ERR_RUNTIME;
+  }
+  if (!h.info[1]) {
+    set_error("SVD did not converge in %d sweeps", h.info[0]);
+    return SEEMPS_ERR_NOCONVERGE;
+  }
+  return SEEMPS_OK;
+}
+
 }  // namespace seemps
 
 extern "C" int64_t seemps_svd_workspace(int64_t m, int64_t n) {
   return seemps::svd_workspace_bytes(m, n);
-}
-
-static int read_info(void* work, int off, int32_t* info, hipStream_t stream) {
-  int32_t h[4] = {0, 0, 0, 0};
-  SEEMPS_HIP(hipMemcpyAsync(h, (char*)work + off, sizeof(h), hipMemcpyDeviceToHost, stream));
-  SEEMPS_HIP(hipStreamSynchronize(stream));
-  if (info) {
-    info[0] = h[0];
-    info[1] = h[1];
-  }
-  if (h[2]) {
-    seemps::set_error("svd: grid barrier timed out (cooperative residency lost)");
-    return SEEMPS_ERR_RUNTIME;
-  }
-  if (!h[1]) {
-    seemps::set_error("SVD did not converge in %d sweeps", h[0]);
-    return SEEMPS_ERR_NOCONVERGE;
-  }
-  return SEEMPS_OK;
 }
 
 extern "C" int seemps_zgesvd(int64_t m, int64_t n, const void* A, int64_t lda, void* U, double* S,
@@ -715,7 +900,7 @@ extern "C" int seemps_zgesvd(int64_t m, int64_t n, const void* A, int64_t lda, v
   int st = seemps::svd_enqueue(m, n, (const double2*)A, lda, (double2*)U, S, (double2*)VH, work,
                                work_bytes, (hipStream_t)stream, &off);
   if (st) return st;
-  return read_info(work, off, info, (hipStream_t)stream);
+  return seemps::svd_read_ctrl(work, off, info, (hipStream_t)stream);
 }
 
 extern "C" int seemps_truncate(int64_t N, const double* s, int method, double tolerance,
@@ -736,7 +921,8 @@ extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, i
                                 int64_t work_bytes, int32_t* info, void* stream) {
   SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd_split: empty matrix");
   SEEMPS_CHECK_ARG(mode == 0 || mode == 1, "svd_split: invalid mode");
-  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_workspace(m, n), "svd_split: workspace too small");
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_workspace(m, n),
+                   "svd_split: workspace too small");
   hipStream_t st_ = (hipStream_t)stream;
   const int64_t r = m < n ? m : n;
   auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
@@ -760,14 +946,16 @@ extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, i
   double herr = 0.0;
   SEEMPS_HIP(hipMemcpyAsync(&hD, Ddev, 8, hipMemcpyDeviceToHost, st_));
   SEEMPS_HIP(hipMemcpyAsync(&herr, edev, 8, hipMemcpyDeviceToHost, st_));
-  if ((st = read_info(p, off, info, st_))) return st;
+  if ((st = seemps::svd_read_ctrl(p, off, info, st_))) return st;
   const int64_t D = hD;
   if (mode == 0) {
-    if ((st = seemps::zscale2d_impl(m, D, nullptr, nullptr, U, r, (double2*)outA, D, st_))) return st;
+    if ((st = seemps::zscale2d_impl(m, D, nullptr, nullptr, U, r, (double2*)outA, D, st_)))
+      return st;
     if ((st = seemps::zscale2d_impl(D, n, S, nullptr, VH, n, (double2*)outB, n, st_))) return st;
   } else {
     if ((st = seemps::zscale2d_impl(m, D, nullptr, S, U, r, (double2*)outA, D, st_))) return st;
-    if ((st = seemps::zscale2d_impl(D, n, nullptr, nullptr, VH, n, (double2*)outB, n, st_))) return st;
+    if ((st = seemps::zscale2d_impl(D, n, nullptr, nullptr, VH, n, (double2*)outB, n, st_)))
+      return st;
   }
   if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, S, r * 8, hipMemcpyDeviceToDevice, st_));
   *D_host = D;

@@ -116,15 +116,18 @@ __device__ __forceinline__ void store_b(double* __restrict__ s_re, double* __res
   }
 }
 
+// Batch index z uses K (the last batch uses Klast: split-K slices share one
+// launch, the final slice being shorter).
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void zgemm_kernel(
-    int64_t M, int64_t N, int64_t K, double2 alpha, const double2* __restrict__ A, int64_t lda,
-    int64_t sA, const double2* __restrict__ B, int64_t ldb, int64_t sB, double2 beta,
-    double2* __restrict__ C, int64_t ldc, int64_t sC) {
+    int64_t M, int64_t N, int64_t Kfull, int64_t Klast, double2 alpha,
+    const double2* __restrict__ A, int64_t lda, int64_t sA, const double2* __restrict__ B,
+    int64_t ldb, int64_t sB, double2 beta, double2* __restrict__ C, int64_t ldc, int64_t sC) {
   __shared__ double as_re[BK * LDA_S], as_im[BK * LDA_S];
   __shared__ double bs_re[BK * LDB_S], bs_im[BK * LDB_S];
 
   const int64_t bz = blockIdx.z;
+  const int64_t K = (bz == (int64_t)gridDim.z - 1) ? Klast : Kfull;
   A += bz * sA;
   B += bz * sB;
   C += bz * sC;
@@ -201,14 +204,90 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
 }
 
 template <int OPA, int OPB>
-void launch(dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K, double2 alpha,
-            const double2* A, int64_t lda, int64_t sA, const double2* B, int64_t ldb, int64_t sB,
-            double2 beta, double2* C, int64_t ldc, int64_t sC) {
-  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA,
-                     B, ldb, sB, beta, C, ldc, sC);
+void launch(dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K, int64_t Klast,
+            double2 alpha, const double2* A, int64_t lda, int64_t sA, const double2* B,
+            int64_t ldb, int64_t sB, double2 beta, double2* C, int64_t ldc, int64_t sC) {
+  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, Klast, alpha, A,
+                     lda, sA, B, ldb, sB, beta, C, ldc, sC);
+}
+
+// C[m, n] = alpha * sum_z P[z][m, n] + beta * C[m, n]  (P packed M x N slices).
+__global__ void splitk_reduce(int64_t M, int64_t N, int nz, const double2* __restrict__ P,
+                              double2 alpha, double2 beta, double2* __restrict__ C, int64_t ldc) {
+  const int64_t total = M * N;
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double2 s = P[e];
+    for (int z = 1; z < nz; ++z) s = cadd(s, P[z * total + e]);
+    const int64_t m = e / N, n = e % N;
+    double2 out = cmul(alpha, s);
+    if (use_beta) out = cfma(beta, C[m * ldc + n], out);
+    C[m * ldc + n] = out;
+  }
+}
+
+int dispatch(int opA, int opB, dim3 grid, hipStream_t stream, int64_t M, int64_t N, int64_t K,
+             int64_t Klast, double2 alpha, const double2* a, int64_t lda, int64_t sA,
+             const double2* b, int64_t ldb, int64_t sB, double2 beta, double2* c, int64_t ldc,
+             int64_t sC) {
+#define SEEMPS_GEMM_CASE(x, y)                                                               \
+  if (opA == x && opB == y) {                                                                \
+    launch<x, y>(grid, stream, M, N, K, Klast, alpha, a, lda, sA, b, ldb, sB, beta, c, ldc, \
+                 sC);                                                                        \
+    return check_launch("zgemm");                                                            \
+  }
+  SEEMPS_GEMM_CASE(0, 0) SEEMPS_GEMM_CASE(0, 1) SEEMPS_GEMM_CASE(0, 2) SEEMPS_GEMM_CASE(0, 3)
+  SEEMPS_GEMM_CASE(1, 0) SEEMPS_GEMM_CASE(1, 1) SEEMPS_GEMM_CASE(1, 2) SEEMPS_GEMM_CASE(1, 3)
+  SEEMPS_GEMM_CASE(2, 0) SEEMPS_GEMM_CASE(2, 1) SEEMPS_GEMM_CASE(2, 2) SEEMPS_GEMM_CASE(2, 3)
+  SEEMPS_GEMM_CASE(3, 0) SEEMPS_GEMM_CASE(3, 1) SEEMPS_GEMM_CASE(3, 2) SEEMPS_GEMM_CASE(3, 3)
+#undef SEEMPS_GEMM_CASE
+  set_error("zgemm: invalid op combination");
+  return SEEMPS_ERR_INVALID;
 }
 
 }  // namespace
+
+// Number of K slices zgemm_ws uses for (M, N, K) (1 = no split).
+int64_t splitk_slices(int64_t M, int64_t N, int64_t K) {
+  const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
+  if (tiles >= 192 || K < 256) return 1;
+  int64_t z = ceil_div(384, tiles);
+  if (z > 8) z = 8;
+  while (z > 1 && K / z < 64) --z;
+  return z;
+}
+
+int64_t splitk_workspace(int64_t M, int64_t N, int64_t K) {
+  const int64_t z = splitk_slices(M, N, K);
+  return z > 1 ? z * M * N : 0;
+}
+
+// zgemm with split-K over the caller's workspace when the output has too few
+// tiles to fill the device (deterministic: slices reduced in a fixed order).
+int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
+             int64_t lda, const void* B, int64_t ldb, double2 beta, void* C, int64_t ldc,
+             double2* ws, int64_t ws_elems, hipStream_t stream) {
+  const int64_t z = splitk_slices(M, N, K);
+  if (z <= 1 || ws == nullptr || ws_elems < z * M * N)
+    return zgemm(opA, opB, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, stream);
+  SEEMPS_CHECK_ARG(ceil_div(M, BM) <= 65535, "zgemm: M too large");
+  int64_t Kc = ceil_div(ceil_div(K, z), BK) * BK;
+  const int64_t nz = ceil_div(K, Kc);
+  const int64_t Klast = K - (nz - 1) * Kc;
+  const int64_t sA = (opA == SEEMPS_OP_N || opA == SEEMPS_OP_C) ? Kc : Kc * lda;
+  const int64_t sB = (opB == SEEMPS_OP_N || opB == SEEMPS_OP_C) ? Kc * ldb : Kc;
+  dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)nz);
+  int st = dispatch(opA, opB, grid, stream, M, N, Kc, Klast, make_double2(1.0, 0.0),
+                    (const double2*)A, lda, sA, (const double2*)B, ldb, sB,
+                    make_double2(0.0, 0.0), ws, N, M * N);
+  if (st) return st;
+  int64_t blocks = ceil_div(M * N, 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, stream, M, N, (int)nz,
+                     ws, alpha, beta, (double2*)C, ldc);
+  return check_launch("splitk_reduce");
+}
 
 // Enqueue a complex GEMM (shared by the other host entry points of the library).
 int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
@@ -226,21 +305,8 @@ int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, cons
                        stream);
   }
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)batch);
-  const double2* a = (const double2*)A;
-  const double2* b = (const double2*)B;
-  double2* c = (double2*)C;
-#define SEEMPS_GEMM_CASE(x, y)                                                              \
-  if (opA == x && opB == y) {                                                               \
-    launch<x, y>(grid, stream, M, N, K, alpha, a, lda, sA, b, ldb, sB, beta, c, ldc, sC); \
-    return check_launch("zgemm");                                                           \
-  }
-  SEEMPS_GEMM_CASE(0, 0) SEEMPS_GEMM_CASE(0, 1) SEEMPS_GEMM_CASE(0, 2) SEEMPS_GEMM_CASE(0, 3)
-  SEEMPS_GEMM_CASE(1, 0) SEEMPS_GEMM_CASE(1, 1) SEEMPS_GEMM_CASE(1, 2) SEEMPS_GEMM_CASE(1, 3)
-  SEEMPS_GEMM_CASE(2, 0) SEEMPS_GEMM_CASE(2, 1) SEEMPS_GEMM_CASE(2, 2) SEEMPS_GEMM_CASE(2, 3)
-  SEEMPS_GEMM_CASE(3, 0) SEEMPS_GEMM_CASE(3, 1) SEEMPS_GEMM_CASE(3, 2) SEEMPS_GEMM_CASE(3, 3)
-#undef SEEMPS_GEMM_CASE
-  set_error("zgemm: unreachable op combination");
-  return SEEMPS_ERR_INVALID;
+  return dispatch(opA, opB, grid, stream, M, N, K, K, alpha, (const double2*)A, lda, sA,
+                  (const double2*)B, ldb, sB, beta, (double2*)C, ldc, sC);
 }
 
 }  // namespace seemps

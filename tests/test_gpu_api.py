@@ -142,8 +142,8 @@ def test_dmrg_parity_small(S):
     rng = np.random.default_rng(3)
     g = ob.random_mps([2] * N, D=4, complex=True, rng=rng)
     H = ob.xxz_mpo(N)
-    ref = oa.dmrg(H, g.copy(), maxiter=10, strategy=oc.Strategy(max_bond_dimension=16))
-    out = S.dmrg(S.MPO(H), dev_mps(S, g), maxiter=10, strategy=S.Strategy(max_bond_dimension=16))
+    ref = oa.dmrg(H, g.copy(), maxiter=10, strategy=oc.Strategy(max_bond_dimension=32))
+    out = S.dmrg(S.MPO(H), dev_mps(S, g), maxiter=10, strategy=S.Strategy(max_bond_dimension=32))
     E0 = ex.ground_energy_sparse(ex.nn_hamiltonian_sparse([ob.xxz_interaction()] * (N - 1),
                                                           [2] * N))
     assert abs(out.energy - E0) < 1e-9

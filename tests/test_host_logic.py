@@ -1,0 +1,97 @@
+"""Host-side logic of the drop-in API on CPU, with the kernels replaced by the
+NumPy test double (tests/fake_kernels.py).  Checks control flow, shapes and
+error bookkeeping against the oracle; the device kernels themselves are
+covered by the gpu-marked tests."""
+
+import numpy as np
+import pytest
+
+from oracle import core as oc, algorithms as oa, builders as ob, exact as ex
+
+
+@pytest.fixture(scope="module")
+def S():
+    import seemps2_amd
+    from seemps2_amd import kernels
+    from tests import fake_kernels as fk
+
+    saved = {}
+    for name in dir(fk):
+        if not name.startswith("_") and hasattr(kernels, name) and callable(getattr(fk, name)):
+            saved[name] = getattr(kernels, name)
+            setattr(kernels, name, getattr(fk, name))
+    yield seemps2_amd
+    for name, f in saved.items():
+        setattr(kernels, name, f)
+
+
+def dev(S, o):
+    return S.MPS(o.tensors, error=o.error)
+
+
+def test_canonical_and_recenter(S, rng):
+    a = ob.random_mps([2] * 7, D=5, complex=True, rng=rng)
+    for c in (0, 3, 6):
+        r = oa.canonical(a, c, oc.Strategy(max_bond_dimension=3))
+        o = S.CanonicalMPS(dev(S, a), center=c, strategy=S.Strategy(max_bond_dimension=3))
+        assert o.bond_dimensions() == r.bond_dimensions()
+        assert abs(o.error() - r.error) < 1e-10
+        assert np.allclose(o.to_vector(), r.to_vector(), atol=1e-12)
+    o.recenter(1)
+    assert o.center == 1
+
+
+def test_apply_simplify(S, rng):
+    N = 8
+    H = ob.heisenberg_mpo(N)
+    a = ob.random_mps([2] * N, D=6, complex=True, rng=rng)
+    ost = oc.Strategy(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=8,
+                      max_sweeps=2)
+    st = S.Strategy(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=8,
+                    max_sweeps=2)
+    r = oa.mpo_apply(H, a, ost, simplify_flag=True)
+    o = S.MPO(H).apply(dev(S, a), strategy=st)
+    assert o.bond_dimensions() == r.bond_dimensions()
+    assert abs(o.error() - r.error) < 1e-10 * max(1, r.error)
+    assert np.allclose(o.to_vector(), r.to_vector(), atol=1e-10)
+    e = oa.mpo_expectation(H, a)
+    assert abs(S.MPO(H).expectation(dev(S, a)) - e) < 1e-12 * abs(e)
+
+
+def test_simplify_sum_and_ops(S, rng):
+    a = ob.random_mps([2] * 6, D=3, complex=True, rng=rng)
+    b = ob.random_mps([2] * 6, D=4, complex=True, rng=rng)
+    st = S.Strategy(tolerance=1e-9, simplification_tolerance=1e-9, normalize=True)
+    r = oa.simplify(oa.OMPSSum([2.0, -1j], [a, b]), oc.Strategy(tolerance=1e-9,
+                    simplification_tolerance=1e-9, normalize=True))
+    s = 2.0 * dev(S, a) - 1j * dev(S, b)
+    o = S.simplify(s, st)
+    assert o.bond_dimensions() == r.bond_dimensions()
+    assert np.allclose(o.to_vector(), r.to_vector(), atol=1e-10)
+    assert abs(o.error() - r.error) < 1e-10
+    assert np.allclose(s.join().to_vector(), oa.join(oa.OMPSSum([2.0, -1j], [a, b])).to_vector())
+
+
+def test_dmrg_host_logic(S):
+    N = 6
+    rng = np.random.default_rng(4)
+    g = ob.random_mps([2] * N, D=2, complex=True, rng=rng)
+    H = ob.xxz_mpo(N)
+    r = oa.dmrg(H, g.copy(), maxiter=6)
+    o = S.dmrg(S.MPO(H), dev(S, g), maxiter=6)
+    assert abs(o.energy - r.energy) < 1e-10
+    assert len(o.trajectory) == len(o.variances)
+    with pytest.raises(ValueError):
+        S.dmrg(S.MPO(H), dev(S, g), maxiter=0)
+    with pytest.raises(ValueError):
+        S.dmrg(S.MPO(H), dev(S, g), tol=-1.0)
+
+
+def test_strategy_api(S):
+    st = S.Strategy(method=S.Truncation.RELATIVE_SINGULAR_VALUE, tolerance=0.0)
+    assert st.get_method() == S.Truncation.ABSOLUTE_SINGULAR_VALUE
+    assert S.NO_TRUNCATION.get_method() == 0 and not S.NO_TRUNCATION.get_simplify_flag()
+    assert S.DEFAULT_STRATEGY.replace(max_bond_dimension=5).get_max_bond_dimension() == 5
+    with pytest.raises(AssertionError):
+        S.Strategy(tolerance=2.0)
+    assert "RelativeNorm" in str(S.DEFAULT_STRATEGY)
