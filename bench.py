@@ -10,9 +10,10 @@ on the device in FP64.  ``value`` counts only the bond updates whose left and
 right bonds are both at the cap, divided by the whole sweep time
 (conservative: the smaller edge updates are timed but not counted).
 
-Also reported: the roofline of the dominant kernel (HIP-event timing of its
-launches inside this process) and a CPU baseline: the NumPy/SciPy oracle
-timed on this host on a bounded sample of the same bond updates.
+Also reported: the roofline of the dominant kernel, from HIP events that the
+library records around each of its launches inside the timed region (on the
+launch stream), and a CPU baseline: the NumPy/SciPy oracle timed on this host
+on a bounded sample of the same bond updates.
 """
 
 from __future__ import annotations
@@ -46,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-timing", default=TIMED_KERNELS,
+                    help="kernels timed with HIP events inside the timed region ('' = none)")
     return ap.parse_args()
 
 
@@ -68,7 +71,10 @@ def grow(QF, strategy, chi, log_every=True):
 def timed_sweeps(QF, strategy, direction, steps, chi):
     """Run `steps` full sweeps; returns (seconds, n_chi_updates, n_updates, matvecs)."""
     import torch
-    from seemps2_amd.optimization import dmrg as D
+    import importlib
+
+    # the package re-exports the dmrg function under the module's name
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
 
     n_chi = n_all = 0
     torch.cuda.synchronize()
@@ -92,45 +98,22 @@ def timed_sweeps(QF, strategy, direction, steps, chi):
     return time.perf_counter() - t0, n_chi, n_all, D.STATS.matvecs, direction, E
 
 
-def kernel_roofline(chi, w=5, d=2, reps=30):
-    """HIP-event timing of the dominant kernels at the chi-sized shapes."""
-    import torch
-    from seemps2_amd import kernels as K
+TIMED_KERNELS = "svd_jacobi,zgemm_kernel"
 
-    g = torch.Generator(device="cuda").manual_seed(1)
 
-    def rnd(*shape):
-        return torch.randn(*shape, dtype=torch.complex128, device="cuda", generator=g)
-
-    s = torch.cuda.current_stream()
-    out = {}
-    # matvec GEMM: L[(a,c), b] @ v[b, (k,l,f)] -> 8 * a*c * b * d*d*f flops
-    L = rnd(chi * w, chi)
-    v = rnd(chi, d * d * chi)
-    C = K.empty(chi * w, d * d * chi)
-    K.gemm(L, v, C=C)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        K.gemm(L, v, C=C)
-    e1.record(s)
-    e1.synchronize()
-    dt = e0.elapsed_time(e1) / reps * 1e-3
-    flops = 8.0 * (chi * w) * chi * (d * d * chi)
-    out["zgemm"] = dict(seconds=dt, flops=flops, shape=f"{chi * w}x{d * d * chi}x{chi}")
-    # truncation SVD of the (d chi) x (d chi) two-site matrix: nominal 88 n^3 real flops
-    n = d * chi
-    A = rnd(n, n)
-    K.svd(A)
-    sreps = 5
-    e0.record(s)
-    for _ in range(sreps):
-        K.svd(A)
-    e1.record(s)
-    e1.synchronize()
-    dts = e0.elapsed_time(e1) / sreps * 1e-3
-    out["svd"] = dict(seconds=dts, flops=88.0 * n ** 3, shape=f"{n}x{n}")
-    return out
+def roofline(records):
+    """Roofline of the dominant kernel from the in-region HIP-event records."""
+    if not records:
+        return None
+    name, r = max(records.items(), key=lambda kv: kv[1]["ms"])
+    secs = r["ms"] * 1e-3
+    achieved = r["flops"] / secs / 1e12 if secs > 0 else 0.0
+    return dict(bound="mfma", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
+                unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 5), traffic=None,
+                kernel=name, launches=r["launches"],
+                avg_launch_us=round(r["ms"] * 1e3 / max(r["launches"], 1), 2),
+                flops_per_launch=round(r["flops"] / max(r["launches"], 1)),
+                share_of_step=None)
 
 
 def cpu_baseline(QF, chi, budget_s):
@@ -208,25 +191,24 @@ def main():
     if args.warmup > 0:
         tw, *_rest, direction, _ = timed_sweeps(QF, strategy, direction, args.warmup, chi)
         log(f"warmup {args.warmup} sweep(s): {tw:.2f} s")
+    from seemps2_amd import _lib
+
+    _lib.timing_enable(args.kernel_timing or None)
     secs, n_chi, n_all, matvecs, direction, E = timed_sweeps(QF, strategy, direction, args.steps,
                                                              chi)
+    records = _lib.timing_records()
+    _lib.timing_enable(None)
     log(f"timed {args.steps} sweep(s): {secs:.3f} s, {n_chi} chi={chi} updates, "
         f"{n_all} total, {matvecs} matvecs, local energy {E:.12f}")
+    for name, r in sorted(records.items(), key=lambda kv: -kv[1]["ms"]):
+        log(f"kernel {name}: {r['launches']} launches, {r['ms']:.2f} ms "
+            f"({100 * r['ms'] * 1e-3 / secs:.1f}% of the timed region), "
+            f"{r['flops'] / max(r['ms'] * 1e-3, 1e-30) / 1e12:.3f} TFLOP/s")
     value = n_chi / secs if secs > 0 else 0.0
-    kr = kernel_roofline(chi)
-    # dominant kernel per bond update: matvec GEMMs (2 per matvec) vs the SVD
     mv_per_update = matvecs / max(n_all, 1)
-    t_gemm = 2 * mv_per_update * kr["zgemm"]["seconds"]
-    t_svd = kr["svd"]["seconds"]
-    dom = "svd" if t_svd > t_gemm else "zgemm"
-    dk = kr[dom]
-    achieved = dk["flops"] / dk["seconds"] / 1e12
-    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / PEAK_FP64_TFLOPS, 4), traffic=None,
-                kernel=("svd_jacobi" if dom == "svd" else "zgemm_kernel<0,0>"),
-                shape=dk["shape"], avg_launch_us=round(dk["seconds"] * 1e6, 1))
-    log(f"kernels: zgemm {kr['zgemm']['shape']} {kr['zgemm']['seconds'] * 1e6:.1f} us, "
-        f"svd {kr['svd']['shape']} {kr['svd']['seconds'] * 1e3:.2f} ms")
+    roof = roofline(records)
+    if roof is not None:
+        roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(QF, chi, args.cpu_seconds)

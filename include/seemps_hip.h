@@ -186,6 +186,25 @@ int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2, 
                                int64_t maxiter, double* stats, void* work,
                                int64_t work_bytes, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Kernel timing (no counterpart in the reference: measurement support for
+ * benchmarks).  The library brackets each launch of the selected kernels with
+ * HIP events on the launch stream and accumulates the elapsed times and the
+ * algorithmic flops of each launch (8 per complex multiply-add; the SVD at
+ * the nominal count 4 * (6 m n^2 + 16 n^3), m >= n).
+ * ------------------------------------------------------------------------- */
+
+/* filter: comma-separated kernel names (e.g. "svd_jacobi,zgemm_kernel"), "*"
+ * for every instrumented kernel, NULL or "" to switch timing off.  Resets the
+ * records. */
+int seemps_timing_enable(const char* filter);
+/* Number of kernels with records.  Waits for the pending events. */
+int seemps_timing_count(void);
+/* Record `index` (< seemps_timing_count()): kernel name (NUL-terminated,
+ * truncated to name_len), launches, summed milliseconds and summed flops. */
+int seemps_timing_get(int index, char* name, int64_t name_len, int64_t* launches,
+                      double* total_ms, double* flops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libseemps_hip.so"
 
 if not LIB_PATH.exists():
     raise ImportError(
-        f"{LIB_PATH} is missing: build it with `python -m seemps2_amd.build` "
+        f"{LIB_PATH} is missing: build it with `python seemps2_amd/build.py` "
         "(the HIP kernels are the only implementation of this package)"
     )
 
@@ -63,6 +63,12 @@ _SIGNATURES = {
     "seemps_two_site_eigensolve": (
         c_int,
         [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _D, _I, POINTER(c_double), _P, _I, _P],
+    ),
+    "seemps_timing_enable": (c_int, [ctypes.c_char_p]),
+    "seemps_timing_count": (c_int, []),
+    "seemps_timing_get": (
+        c_int,
+        [c_int, ctypes.c_char_p, _I, POINTER(c_int64), POINTER(c_double), POINTER(c_double)],
     ),
 }
 
@@ -123,3 +129,20 @@ class _Workspace:
 
 
 workspace = _Workspace()
+
+
+def timing_enable(kernels: str | None) -> None:
+    """Time the named kernels' launches with HIP events ("*" = all, None = off)."""
+    check(lib.seemps_timing_enable(kernels.encode() if kernels else None))
+
+
+def timing_records() -> dict[str, dict]:
+    """{kernel: {launches, ms, flops}} accumulated since timing_enable."""
+    out = {}
+    buf = ctypes.create_string_buffer(128)
+    for i in range(lib.seemps_timing_count()):
+        n, ms, fl = c_int64(), c_double(), c_double()
+        check(lib.seemps_timing_get(i, buf, 128, ctypes.byref(n), ctypes.byref(ms),
+                                    ctypes.byref(fl)))
+        out[buf.value.decode()] = dict(launches=n.value, ms=ms.value, flops=fl.value)
+    return out

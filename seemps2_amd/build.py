@@ -1,7 +1,9 @@
 """Build libseemps_hip.so (gfx950) in-tree with hipcc.
 
 Every ``csrc/*.hip`` file is compiled to an object for ``--offload-arch=gfx950``
-and linked into ``seemps2_amd/libseemps_hip.so`` next to this file.
+and linked into ``seemps2_amd/libseemps_hip.so`` next to this file.  Run it
+as ``python seemps2_amd/build.py`` (by path: importing the package would load
+the library being rebuilt).
 """
 
 from __future__ import annotations

@@ -38,6 +38,22 @@ void set_error(const char* fmt, ...);
 // Check the most recent launch.
 int check_launch(const char* what);
 
+// Brackets one launch with HIP events when timing is enabled for `name`
+// (seemps_timing_enable): construct right before the launch, on its stream.
+class TimedLaunch {
+ public:
+  TimedLaunch(const char* name, hipStream_t stream, double flops);
+  ~TimedLaunch();
+  TimedLaunch(const TimedLaunch&) = delete;
+  TimedLaunch& operator=(const TimedLaunch&) = delete;
+
+ private:
+  int slot_;
+  hipEvent_t start_;
+  hipStream_t stream_;
+  double flops_;
+};
+
 struct cplx {
   double re, im;
 };

@@ -597,12 +597,26 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
       // Two classical Gram-Schmidt passes against V[0..j], then the norm:
       // four launches (dots | subtract + dots | subtract + norm | normalise).
       const int kk = (int)(j + 1);
-      hipLaunchKernelGGL(lz_dots, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part);
-      hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part, w.part2,
-                         w.pnorm, w.sc->h, 0);
-      hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part2, w.part,
-                         w.pnorm, w.sc->h2, 1);
-      hipLaunchKernelGGL(lz_finish, dim3(gb), dim3(256), 0, st_, n, nbk, (int)j, w.pnorm, w.sc, wv);
+      const double orth = 8.0 * n * kk;
+      {
+        TimedLaunch timed("lz_dots", st_, orth);
+        hipLaunchKernelGGL(lz_dots, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part);
+      }
+      {
+        TimedLaunch timed("lz_sub", st_, 2.0 * orth);
+        hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part, w.part2,
+                           w.pnorm, w.sc->h, 0);
+      }
+      {
+        TimedLaunch timed("lz_sub", st_, 2.0 * orth);
+        hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part2, w.part,
+                           w.pnorm, w.sc->h2, 1);
+      }
+      {
+        TimedLaunch timed("lz_finish", st_, 0.0);
+        hipLaunchKernelGGL(lz_finish, dim3(gb), dim3(256), 0, st_, n, nbk, (int)j, w.pnorm, w.sc,
+                           wv);
+      }
       if ((st = check_launch("lanczos step"))) return st;
     }
     hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)m);

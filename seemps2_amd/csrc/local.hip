@@ -137,6 +137,7 @@ int zlocal_impl(int64_t P, int64_t I, int64_t Q, int64_t J1, int64_t J2, const d
   const int64_t nblocks = P * nqb;
   SEEMPS_CHECK_ARG(nblocks <= 0x7fffffff, "zlocal: grid too large");
   const size_t shmem = (size_t)(I * LQ + I * J1 * J2) * sizeof(double2);
+  TimedLaunch timed("zlocal_kernel", stream, 8.0 * P * I * Q * J1 * J2);
   hipLaunchKernelGGL(zlocal_kernel, dim3((unsigned)nblocks), dim3(256), shmem, stream, P, (int)I,
                      Q, (int)J1, (int)J2, nqb, in, Mop, out, accumulate);
   return check_launch("zlocal");

@@ -144,6 +144,126 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(double2* __restric
   for (int e = tid; e < nb * nb; e += blockDim.x) T[e] = s_T[e];
 }
 
+// Panel factorisation with the whole panel resident in LDS (nb <= 16 columns
+// of L <= QR_LDS_ROWS rows): one load, four barriers per column (norm |
+// reflector | one wavefront per column dot product | rank-1 update), the T
+// factor from wavefront-parallel dot products, one write-back.  Same outputs
+// as qr_panel_kernel.
+constexpr int QR_LDS_THREADS = 1024;
+constexpr int QR_LDS_ROWS = 560;
+constexpr int QR_LDS_NB = 16;
+
+__global__ __launch_bounds__(QR_LDS_THREADS) void qr_panel_lds_kernel(
+    double2* __restrict__ A, int64_t len, int64_t j0, int nb, double2* __restrict__ tau,
+    double2* __restrict__ Vp, double2* __restrict__ T) {
+  __shared__ double2 P[QR_LDS_NB * QR_LDS_ROWS];  // column j at P + j * L
+  __shared__ double2 s_T[QR_LDS_NB * QR_LDS_NB];
+  __shared__ double2 s_y[QR_LDS_NB * QR_LDS_NB];  // y[i][j] = V_i^H V_j, i < j
+  __shared__ double2 s_coef[QR_LDS_NB];
+  __shared__ double2 s_tau[QR_LDS_NB];
+  __shared__ double s_red[QR_LDS_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwaves = QR_LDS_THREADS / 64;
+  const int L = (int)(len - j0);
+  for (int e = tid; e < nb * L; e += QR_LDS_THREADS) {
+    const int j = e / L, rr = e % L;
+    P[e] = A[(j0 + j) * len + j0 + rr];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    double2* col = P + j * L;
+    double acc = 0.0;
+    for (int rr = j + 1 + tid; rr < L; rr += QR_LDS_THREADS) acc += cabs2(col[rr]);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) s_red[wave] = acc;
+    const double2 alpha = col[j];
+    __syncthreads();  // #1: partial norms visible, alpha read by everyone
+    double xnorm2 = 0.0;
+    for (int q = 0; q < nwaves; ++q) xnorm2 += s_red[q];
+    double2 t = make_double2(0.0, 0.0);
+    double beta = alpha.x;
+    double2 scal = make_double2(0.0, 0.0);
+    const bool reflect = !(xnorm2 == 0.0 && alpha.y == 0.0);
+    if (reflect) {
+      const double nrm = sqrt(cabs2(alpha) + xnorm2);
+      beta = (alpha.x >= 0.0) ? -nrm : nrm;
+      t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double2 d = make_double2(alpha.x - beta, alpha.y);
+      const double dd = cabs2(d);
+      scal = make_double2(d.x / dd, -d.y / dd);
+      for (int rr = j + 1 + tid; rr < L; rr += QR_LDS_THREADS) col[rr] = cmul(scal, col[rr]);
+    }
+    if (tid == 0) {
+      col[j] = make_double2(beta, 0.0);
+      s_tau[j] = t;
+      tau[j0 + j] = t;
+    }
+    __syncthreads();  // #2: v_j complete
+    if (reflect) {
+      // w_jj = v^H P[jj] (v[j] = 1), coef = conj(t) w
+      for (int jj = j + 1 + wave; jj < nb; jj += nwaves) {
+        const double2* cc = P + jj * L;
+        double2 a = make_double2(0.0, 0.0);
+        for (int rr = j + 1 + lane; rr < L; rr += 64) a = cfmac(col[rr], cc[rr], a);
+        for (int off = 32; off > 0; off >>= 1) {
+          a.x += __shfl_down(a.x, off, 64);
+          a.y += __shfl_down(a.y, off, 64);
+        }
+        if (lane == 0) s_coef[jj] = cmul(make_double2(t.x, -t.y), cadd(a, cc[j]));
+      }
+      __syncthreads();  // #3
+      const int rows = L - j;
+      const int ncols = nb - j - 1;
+      for (int e = tid; e < rows * ncols; e += QR_LDS_THREADS) {
+        const int jj = j + 1 + e / rows, rr = j + e % rows;
+        const double2 v = (rr == j) ? make_double2(1.0, 0.0) : col[rr];
+        const double2 p = cmul(s_coef[jj], v);
+        double2* cc = P + jj * L;
+        cc[rr] = make_double2(cc[rr].x - p.x, cc[rr].y - p.y);
+      }
+      __syncthreads();  // #4
+    }
+  }
+  // y[i][j] = conj(V_i[j]) + sum_{rr > j} conj(V_i[rr]) V_j[rr], i < j
+  for (int q = wave; q < nb * nb; q += nwaves) {
+    const int i = q / nb, j = q % nb;
+    if (i >= j) continue;
+    const double2* ci = P + i * L;
+    const double2* cj = P + j * L;
+    double2 a = make_double2(0.0, 0.0);
+    for (int rr = j + 1 + lane; rr < L; rr += 64) a = cfmac(ci[rr], cj[rr], a);
+    for (int off = 32; off > 0; off >>= 1) {
+      a.x += __shfl_down(a.x, off, 64);
+      a.y += __shfl_down(a.y, off, 64);
+    }
+    if (lane == 0) s_y[i * QR_LDS_NB + j] = cadd(a, make_double2(ci[j].x, -ci[j].y));
+  }
+  for (int e = tid; e < QR_LDS_NB * QR_LDS_NB; e += QR_LDS_THREADS) s_T[e] = make_double2(0, 0);
+  __syncthreads();
+  // T (zlarft, forward, columnwise): T[j][j] = tau_j,
+  // T[0:j, j] = -tau_j * T[0:j, 0:j] * y[0:j, j].
+  for (int j = 0; j < nb; ++j) {
+    if (tid < j) {
+      const int i = tid;
+      double2 acc2 = make_double2(0.0, 0.0);
+      for (int k = i; k < j; ++k) acc2 = cfma(s_T[i * QR_LDS_NB + k], s_y[k * QR_LDS_NB + j], acc2);
+      const double2 v = cmul(s_tau[j], acc2);
+      s_T[i * QR_LDS_NB + j] = make_double2(-v.x, -v.y);
+    } else if (tid == j) {
+      s_T[j * QR_LDS_NB + j] = s_tau[j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * nb; e += QR_LDS_THREADS)
+    T[e] = s_T[(e / nb) * QR_LDS_NB + e % nb];
+  for (int e = tid; e < nb * L; e += QR_LDS_THREADS) {
+    const int j = e / L, rr = e % L;
+    const double2 p = P[e];
+    A[(j0 + j) * len + j0 + rr] = p;
+    Vp[e] = (rr < j) ? make_double2(0.0, 0.0) : (rr == j) ? make_double2(1.0, 0.0) : p;
+  }
+}
+
 }  // namespace
 
 // In-place blocked QR of A_cm (ncol columns of length len, len >= ncol).
@@ -158,9 +278,17 @@ int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, 
     const int64_t L = len - j0;
     double2* Vp = Vall + j0 * len;  // nb x L block (fits: nb * L <= nb * len)
     double2* T = Tall + p * nb_max * nb_max;
-    hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, stream, A, len, j0, nb, tau,
-                       Vp, T);
-    if ((st = check_launch("qr_panel"))) return st;
+    if (L <= QR_LDS_ROWS && nb <= QR_LDS_NB) {
+      TimedLaunch timed("qr_panel_lds_kernel", stream, 0.0);
+      hipLaunchKernelGGL(qr_panel_lds_kernel, dim3(1), dim3(QR_LDS_THREADS), 0, stream, A, len,
+                         j0, nb, tau, Vp, T);
+      if ((st = check_launch("qr_panel_lds"))) return st;
+    } else {
+      TimedLaunch timed("qr_panel_kernel", stream, 0.0);
+      hipLaunchKernelGGL(qr_panel_kernel, dim3(1), dim3(QR_THREADS), 0, stream, A, len, j0, nb,
+                         tau, Vp, T);
+      if ((st = check_launch("qr_panel"))) return st;
+    }
     const int64_t ntr = ncol - j0 - nb;
     if (ntr <= 0) continue;
     double2* At = A + (j0 + nb) * len + j0;  // ntr x L, row stride len

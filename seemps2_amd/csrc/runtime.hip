@@ -1,5 +1,10 @@
 // Error reporting, device queries and the ABI version of libseemps_hip.so.
+#include <algorithm>
+#include <atomic>
 #include <cstdarg>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "common.h"
 
@@ -40,8 +45,152 @@ int device_info(DeviceInfo* info) {
   return SEEMPS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Kernel timing registry.
+namespace {
+
+struct TimingSlot {
+  std::string name;
+  int64_t launches = 0;
+  double ms = 0.0;
+  double flops = 0.0;
+};
+struct PendingTiming {
+  int slot;
+  hipEvent_t start, stop;
+  double flops;
+};
+
+std::atomic<bool> g_timing_on{false};
+std::mutex g_timing_mu;
+bool g_timing_all = false;
+std::vector<std::string> g_timing_filter;
+std::vector<TimingSlot> g_slots;
+std::vector<PendingTiming> g_pending;
+
+// Caller holds g_timing_mu.
+int timing_slot(const char* name) {
+  if (!g_timing_all) {
+    bool wanted = false;
+    for (const auto& f : g_timing_filter) wanted = wanted || (f == name);
+    if (!wanted) return -1;
+  }
+  for (size_t i = 0; i < g_slots.size(); ++i)
+    if (g_slots[i].name == name) return (int)i;
+  TimingSlot s;
+  s.name = name;
+  g_slots.push_back(s);
+  return (int)g_slots.size() - 1;
+}
+
+// Caller holds g_timing_mu.
+int drain_pending() {
+  int st = SEEMPS_OK;
+  for (auto& p : g_pending) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(p.stop) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      g_slots[p.slot].launches += 1;
+      g_slots[p.slot].ms += ms;
+      g_slots[p.slot].flops += p.flops;
+    } else {
+      set_error("kernel timing: event query failed");
+      st = SEEMPS_ERR_RUNTIME;
+    }
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  g_pending.clear();
+  return st;
+}
+
+}  // namespace
+
+TimedLaunch::TimedLaunch(const char* name, hipStream_t stream, double flops)
+    : slot_(-1), start_(nullptr), stream_(stream), flops_(flops) {
+  if (!g_timing_on.load(std::memory_order_relaxed)) return;
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  const int slot = timing_slot(name);
+  if (slot < 0) return;
+  if (hipEventCreate(&start_) != hipSuccess) return;
+  if (hipEventRecord(start_, stream_) != hipSuccess) {
+    (void)hipEventDestroy(start_);
+    return;
+  }
+  slot_ = slot;
+}
+
+TimedLaunch::~TimedLaunch() {
+  if (slot_ < 0) return;
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess) {
+    (void)hipEventDestroy(start_);
+    return;
+  }
+  if (hipEventRecord(stop, stream_) != hipSuccess) {
+    (void)hipEventDestroy(start_);
+    (void)hipEventDestroy(stop);
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  g_pending.push_back(PendingTiming{slot_, start_, stop, flops_});
+  // bound the number of live events on long runs
+  if (g_pending.size() >= 8192) (void)drain_pending();
+}
+
 }  // namespace seemps
 
 extern "C" const char* seemps_last_error(void) { return seemps::g_last_error; }
 
 extern "C" int seemps_abi_version(void) { return 1; }
+
+extern "C" int seemps_timing_enable(const char* filter) {
+  using namespace seemps;
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  const int st = drain_pending();
+  g_slots.clear();
+  g_timing_filter.clear();
+  g_timing_all = false;
+  if (!filter || !*filter) {
+    g_timing_on.store(false);
+    return st;
+  }
+  std::string f(filter);
+  size_t pos = 0;
+  while (pos <= f.size()) {
+    size_t comma = f.find(',', pos);
+    if (comma == std::string::npos) comma = f.size();
+    std::string item = f.substr(pos, comma - pos);
+    if (item == "*") g_timing_all = true;
+    if (!item.empty()) g_timing_filter.push_back(item);
+    pos = comma + 1;
+  }
+  g_timing_on.store(true);
+  return st;
+}
+
+extern "C" int seemps_timing_count(void) {
+  using namespace seemps;
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  (void)drain_pending();
+  return (int)g_slots.size();
+}
+
+extern "C" int seemps_timing_get(int index, char* name, int64_t name_len, int64_t* launches,
+                                 double* total_ms, double* flops) {
+  using namespace seemps;
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  int st = drain_pending();
+  if (st) return st;
+  SEEMPS_CHECK_ARG(index >= 0 && index < (int)g_slots.size(), "timing: index out of range");
+  const TimingSlot& s = g_slots[index];
+  if (name && name_len > 0) {
+    const size_t n = std::min((size_t)(name_len - 1), s.name.size());
+    memcpy(name, s.name.data(), n);
+    name[n] = '\0';
+  }
+  if (launches) *launches = s.launches;
+  if (total_ms) *total_ms = s.ms;
+  if (flops) *flops = s.flops;
+  return SEEMPS_OK;
+}

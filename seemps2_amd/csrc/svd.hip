@@ -58,6 +58,7 @@ struct Ctrl {
   long long prof[8];          // wall-clock ticks of workgroup 0 per phase
   double frob2;               // ||W||_F^2 (invariant under the rotations)
   double tiny2;               // squared norm below which a column is negligible
+  double offhist[64];         // off-diagonal measure of each sweep (diagnostics)
 };
 
 struct SvdShape {
@@ -274,91 +275,124 @@ __device__ __forceinline__ int64_t pair_col(int64_t bi, int64_t bj, int t) {
   return (t < BW) ? bi * BW + t : bj * BW + (t - BW);
 }
 
+// Rotation of the index pair (a, b) of the Hermitian matrix G (row stride GP)
+// that annihilates G[a][b]: J = [[cs, sn ph], [-sn conj(ph), cs]], applied as
+// G <- J^H G J.  Identity (sn = 0) when the pair is already orthogonal to
+// tolerance or either column is negligible.
+__device__ __forceinline__ void jacobi_rotation(const double2* G, int a, int b, double tol,
+                                                double tiny2, double* cs, double* sn,
+                                                double2* ph) {
+  const double gaa = G[a * GP + a].x, gbb = G[b * GP + b].x;
+  const double2 c = G[a * GP + b];
+  const double ac = sqrt(cabs2(c));
+  *cs = 1.0;
+  *sn = 0.0;
+  *ph = make_double2(1.0, 0.0);
+  if (ac > 0.0 && gaa > tiny2 && gbb > tiny2 && ac > tol * sqrt(gaa) * sqrt(gbb)) {
+    *ph = make_double2(c.x / ac, c.y / ac);
+    const double tau = (gbb - gaa) / (2.0 * ac);
+    double t_;
+    if (fabs(tau) > 1e150)
+      t_ = 0.5 / tau;
+    else
+      t_ = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    *cs = 1.0 / sqrt(1.0 + t_ * t_);
+    *sn = t_ * (*cs);
+  }
+}
+
+// Index pair t (0 <= t < PW/2) of round k of the circle schedule over PW indices.
+__device__ __forceinline__ void inner_pair(int k, int t, int* a, int* b) {
+  auto idx = [&](int x) { return 1 + (x + k) % (PW - 1); };
+  if (t == 0) {
+    *a = 0;
+    *b = idx(0);
+  } else {
+    *a = idx(t);
+    *b = idx(PW - 1 - t);
+  }
+}
+
 // Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS, row
-// stride GP), accumulating Q (LDS, on entry identity).  256 threads.  Pairs
-// with a negligible column (squared norm <= tiny2) are not rotated.
-__device__ void inner_jacobi(double2* G, double2* Q, double2* rot, double tol, double tiny2,
-                             int sweeps) {
+// stride GP; G2 is a second buffer of the same shape), accumulating Q (LDS,
+// on entry identity).  256 threads: in each round thread (p, q) applies
+// J_p^H G[pair p, pair q] J_q to one 2x2 block, reading G and writing the other
+// buffer, and updates two rows of Q for pair p; every thread derives the two
+// rotations it needs itself, so a round costs one barrier.  Returns the buffer
+// holding the final G.
+__device__ double2* inner_jacobi(double2* G, double2* G2, double2* Q, double tol, double tiny2,
+                                 int sweeps) {
   const int tid = threadIdx.x;
+  const int p = tid >> 4, q = tid & 15;
+  double2* cur = G;
+  double2* nxt = G2;
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int k = 0; k < PW - 1; ++k) {
-      if (tid < PW / 2) {
-        const int t = tid;
-        int a, b;
-        auto idx = [&](int x) { return 1 + (x + k) % (PW - 1); };
-        if (t == 0) {
-          a = 0;
-          b = idx(0);
-        } else {
-          a = idx(t);
-          b = idx(PW - 1 - t);
-        }
-        const double gaa = G[a * GP + a].x, gbb = G[b * GP + b].x;
-        const double2 c = G[a * GP + b];
-        const double ac = sqrt(cabs2(c));
-        double cs = 1.0, sn = 0.0;
-        double2 ph = make_double2(1.0, 0.0);
-        if (ac > 0.0 && gaa > tiny2 && gbb > tiny2 && ac > tol * sqrt(gaa) * sqrt(gbb)) {
-          ph = make_double2(c.x / ac, c.y / ac);
-          const double tau = (gbb - gaa) / (2.0 * ac);
-          double t_;
-          if (fabs(tau) > 1e150)
-            t_ = 0.5 / tau;
-          else
-            t_ = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          cs = 1.0 / sqrt(1.0 + t_ * t_);
-          sn = t_ * cs;
-        }
-        rot[3 * t + 0] = make_double2(cs, sn);
-        rot[3 * t + 1] = ph;
-        rot[3 * t + 2] = make_double2((double)a, (double)b);
+      int a, b, c, d;
+      inner_pair(k, p, &a, &b);
+      inner_pair(k, q, &c, &d);
+      double csp, snp, csq, snq;
+      double2 php, phq;
+      jacobi_rotation(cur, a, b, tol, tiny2, &csp, &snp, &php);
+      if (p == q) {
+        csq = csp;
+        snq = snp;
+        phq = php;
+      } else {
+        jacobi_rotation(cur, c, d, tol, tiny2, &csq, &snq, &phq);
       }
-      __syncthreads();
-      // Column pass: G <- G J, Q <- Q J, J = [[cs, sn e^{i phi}], [-sn e^{-i phi}, cs]].
-      for (int e = tid; e < 2 * PW * (PW / 2); e += blockDim.x) {
-        const int which = e / (PW * (PW / 2));  // 0: G, 1: Q
-        const int rem = e % (PW * (PW / 2));
-        const int x = rem / (PW / 2), t = rem % (PW / 2);  // lanes of a row share x
-        const double cs = rot[3 * t].x, sn = rot[3 * t].y;
-        if (sn == 0.0) continue;
-        const double2 ph = rot[3 * t + 1];
-        const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
-        double2* M = which ? Q : G;
-        const double2 xa = M[x * GP + a], xb = M[x * GP + b];
-        const double2 phc = make_double2(ph.x, -ph.y);
-        const double2 tb = cmul(phc, xb);
-        const double2 ta = cmul(ph, xa);
-        M[x * GP + a] = make_double2(cs * xa.x - sn * tb.x, cs * xa.y - sn * tb.y);
-        M[x * GP + b] = make_double2(sn * ta.x + cs * xb.x, sn * ta.y + cs * xb.y);
+      // Y = X J_q on the block rows (a, b), columns (c, d)
+      const double2 xac = cur[a * GP + c], xad = cur[a * GP + d];
+      const double2 xbc = cur[b * GP + c], xbd = cur[b * GP + d];
+      const double2 pqc = make_double2(phq.x, -phq.y);
+      auto right = [&](double2 xc, double2 xd, double2* yc, double2* yd) {
+        const double2 t1 = cmul(pqc, xd), t2 = cmul(phq, xc);
+        *yc = make_double2(csq * xc.x - snq * t1.x, csq * xc.y - snq * t1.y);
+        *yd = make_double2(snq * t2.x + csq * xd.x, snq * t2.y + csq * xd.y);
+      };
+      double2 yac, yad, ybc, ybd;
+      right(xac, xad, &yac, &yad);
+      right(xbc, xbd, &ybc, &ybd);
+      // Z = J_p^H Y
+      const double2 ppc = make_double2(php.x, -php.y);
+      auto left = [&](double2 ya, double2 yb, double2* za, double2* zb) {
+        const double2 t1 = cmul(php, yb), t2 = cmul(ppc, ya);
+        *za = make_double2(csp * ya.x - snp * t1.x, csp * ya.y - snp * t1.y);
+        *zb = make_double2(snp * t2.x + csp * yb.x, snp * t2.y + csp * yb.y);
+      };
+      double2 zac, zbc, zad, zbd;
+      left(yac, ybc, &zac, &zbc);
+      left(yad, ybd, &zad, &zbd);
+      if (p == q && snp != 0.0) {
+        // (c, d) == (a, b): the rotated pair is exactly orthogonal
+        zad = make_double2(0.0, 0.0);
+        zbc = make_double2(0.0, 0.0);
+        zac.y = 0.0;
+        zbd.y = 0.0;
       }
-      __syncthreads();
-      // Row pass: G <- J^H G.
-      for (int e = tid; e < PW * (PW / 2); e += blockDim.x) {
-        const int t = e / PW, y = e % PW;
-        const double cs = rot[3 * t].x, sn = rot[3 * t].y;
-        if (sn == 0.0) continue;
-        const double2 ph = rot[3 * t + 1];
-        const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
-        const double2 ay = G[a * GP + y], by = G[b * GP + y];
-        const double2 phc = make_double2(ph.x, -ph.y);
-        const double2 t1 = cmul(ph, by), t2 = cmul(phc, ay);
-        G[a * GP + y] = make_double2(cs * ay.x - sn * t1.x, cs * ay.y - sn * t1.y);
-        G[b * GP + y] = make_double2(sn * t2.x + cs * by.x, sn * t2.y + cs * by.y);
-      }
-      __syncthreads();
-      if (tid < PW / 2) {
-        const int t = tid;
-        if (rot[3 * t].y != 0.0) {
-          const int a = (int)rot[3 * t + 2].x, b = (int)rot[3 * t + 2].y;
-          G[a * GP + b] = make_double2(0.0, 0.0);
-          G[b * GP + a] = make_double2(0.0, 0.0);
-          G[a * GP + a].y = 0.0;
-          G[b * GP + b].y = 0.0;
+      nxt[a * GP + c] = zac;
+      nxt[a * GP + d] = zad;
+      nxt[b * GP + c] = zbc;
+      nxt[b * GP + d] = zbd;
+      // Q <- Q J_p on rows 2q, 2q + 1
+      if (snp != 0.0) {
+        const double2 ppc2 = make_double2(php.x, -php.y);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int x = 2 * q + u;
+          const double2 xa = Q[x * GP + a], xb = Q[x * GP + b];
+          const double2 tb = cmul(ppc2, xb), ta = cmul(php, xa);
+          Q[x * GP + a] = make_double2(csp * xa.x - snp * tb.x, csp * xa.y - snp * tb.y);
+          Q[x * GP + b] = make_double2(snp * ta.x + csp * xb.x, snp * ta.y + csp * xb.y);
         }
       }
       __syncthreads();
+      double2* tmp = cur;
+      cur = nxt;
+      nxt = tmp;
     }
   }
+  return cur;
 }
 
 // Rotate rows [r0, r1) of the PW columns (bi, bj) of a column-major matrix M
@@ -392,8 +426,8 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
                                                   int max_sweeps) {
   __shared__ double2 tile[PW * RTP];
   __shared__ double2 G[PW * GP];
+  __shared__ double2 G2[PW * GP];
   __shared__ double2 Q[PW * GP];
-  __shared__ double2 rot[3 * (PW / 2)];
   __shared__ int s_flag;
   __shared__ double s_off[8];
 
@@ -505,7 +539,7 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
           pt[2] += t1 - t0;
           t0 = t1;
         }
-        inner_jacobi(G, Q, rot, tol, tiny2, inner_sweeps);
+        inner_jacobi(G, G2, Q, tol, tiny2, inner_sweeps);
         if (prof) {
           const long long t1 = now_ticks();
           pt[3] += t1 - t0;
@@ -525,7 +559,9 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
     {
       const unsigned long long bits =
           __hip_atomic_load(&w.ctrl->off[sweep & 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      if (__longlong_as_double((long long)bits) <= tol) {
+      const double off_sweep = __longlong_as_double((long long)bits);
+      if (blockIdx.x == 0 && tid == 0 && sweep < 64) w.ctrl->offhist[sweep] = off_sweep;
+      if (off_sweep <= tol) {
         converged = 1;
         ++sweep;
         break;
@@ -611,11 +647,17 @@ __global__ void svd_output(SvdShape s, SvdWork w, double2* __restrict__ U, doubl
 
 // Complete the W-side singular vectors of negligible singular values
 // (sigma^2 <= tiny2: at the rounding level of the factorisation, where the
-// directions are not determined) to an orthonormal set, by classical
-// Gram-Schmidt twice against unit candidates.  The singular values themselves
-// are kept.  One workgroup; loops over nothing in the common case.
-__global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, double2* __restrict__ U,
-                                                    double2* __restrict__ VH) {
+// directions are not determined) to an orthonormal set.  The singular values
+// themselves are kept.  For slot k the candidate is the unit vector e_x whose
+// coordinate x has the smallest leverage sum_{j<k} |u_j[x]|^2 among the
+// vectors already fixed; its residual after projection then has squared norm
+// 1 - leverage >= 1 - k/len > 0, so the completion cannot fail.  Projection is
+// classical Gram-Schmidt applied twice, one wavefront per dot product.  Only
+// slots k < *kmax are completed when kmax is given (the caller keeps no more).
+// One workgroup of 1024 threads; coef holds at least ncol complex values.
+__global__ __launch_bounds__(1024) void svd_complete(SvdShape s, SvdWork w, double2* __restrict__ U,
+                                                     double2* __restrict__ VH,
+                                                     const int64_t* __restrict__ kmax) {
   const int64_t r = s.ncol, len = s.len;
   const int wside_u = (s.use_qr == s.transposed);
   // element x of W-side vector k, as a plain (unconjugated) vector component
@@ -630,61 +672,96 @@ __global__ __launch_bounds__(256) void svd_complete(SvdShape s, SvdWork w, doubl
     else
       VH[k * s.n + x] = make_double2(v.x, -v.y);
   };
-  __shared__ double2 red[256];
-  __shared__ double2 coef;
-  int64_t cand = 0;
+  __shared__ double s_val[16];
+  __shared__ int64_t s_idx[16];
+  __shared__ int64_t s_best;
+  __shared__ double s_nrm;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwaves = (int)(blockDim.x >> 6);
+  double2* coef = w.part;
   const double tiny2 = w.ctrl->tiny2;
-  for (int64_t k = 0; k < r; ++k) {
+  int64_t kend = r;
+  if (kmax && *kmax < kend) kend = *kmax;
+  for (int64_t k = 0; k < kend; ++k) {
     const double sg = w.sigma[w.perm[k]];
     if (sg > 0.0 && sg * sg > tiny2) continue;
-    for (; cand < len; ++cand) {
-      for (int64_t x = threadIdx.x; x < len; x += blockDim.x)
-        store(k, x, make_double2(x == cand ? 1.0 : 0.0, 0.0));
-      __syncthreads();
-      for (int pass = 0; pass < 2; ++pass) {
-        for (int64_t j = 0; j < k; ++j) {
-          double2 acc = make_double2(0.0, 0.0);
-          for (int64_t x = threadIdx.x; x < len; x += blockDim.x)
-            acc = cfmac(load(j, x), load(k, x), acc);
-          red[threadIdx.x] = acc;
-          __syncthreads();
-          if (threadIdx.x == 0) {
-            double2 t = make_double2(0, 0);
-            for (int q = 0; q < (int)blockDim.x; ++q) t = cadd(t, red[q]);
-            coef = t;
-          }
-          __syncthreads();
-          const double2 cf = coef;
-          for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
-            const double2 pa = cmul(cf, load(j, x));
-            const double2 b = load(k, x);
-            store(k, x, make_double2(b.x - pa.x, b.y - pa.y));
-          }
-          __syncthreads();
-        }
-      }
-      double acc = 0.0;
-      for (int64_t x = threadIdx.x; x < len; x += blockDim.x) acc += cabs2(load(k, x));
-      red[threadIdx.x] = make_double2(acc, 0.0);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double t = 0;
-        for (int q = 0; q < (int)blockDim.x; ++q) t += red[q].x;
-        coef = make_double2(sqrt(t), 0.0);
-      }
-      __syncthreads();
-      const double nrm = coef.x;
-      __syncthreads();
-      if (nrm > 0.5) {
-        for (int64_t x = threadIdx.x; x < len; x += blockDim.x) {
-          const double2 b = load(k, x);
-          store(k, x, make_double2(b.x / nrm, b.y / nrm));
-        }
-        __syncthreads();
-        ++cand;
-        break;
+    // Candidate coordinate: smallest leverage, ties to the smallest index.
+    double best = 1.0e300;
+    int64_t bx = len;
+    for (int64_t x = tid; x < len; x += blockDim.x) {
+      double lev = 0.0;
+      for (int64_t j = 0; j < k; ++j) lev += cabs2(load(j, x));
+      if (lev < best) {
+        best = lev;
+        bx = x;
       }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_down(best, off, 64);
+      const int64_t ox = __shfl_down(bx, off, 64);
+      if (ov < best || (ov == best && ox < bx)) {
+        best = ov;
+        bx = ox;
+      }
+    }
+    if (lane == 0) {
+      s_val[wave] = best;
+      s_idx[wave] = bx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double bv = s_val[0];
+      int64_t bi = s_idx[0];
+      for (int q = 1; q < nwaves; ++q)
+        if (s_val[q] < bv || (s_val[q] == bv && s_idx[q] < bi)) {
+          bv = s_val[q];
+          bi = s_idx[q];
+        }
+      s_best = bi;
+    }
+    __syncthreads();
+    const int64_t xb = s_best;
+    for (int64_t x = tid; x < len; x += blockDim.x)
+      store(k, x, make_double2(x == xb ? 1.0 : 0.0, 0.0));
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int64_t j = wave; j < k; j += nwaves) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int64_t x = lane; x < len; x += 64) acc = cfmac(load(j, x), load(k, x), acc);
+        for (int off = 32; off > 0; off >>= 1) {
+          acc.x += __shfl_down(acc.x, off, 64);
+          acc.y += __shfl_down(acc.y, off, 64);
+        }
+        if (lane == 0) coef[j] = acc;
+      }
+      __syncthreads();
+      for (int64_t x = tid; x < len; x += blockDim.x) {
+        double2 b = load(k, x);
+        for (int64_t j = 0; j < k; ++j) {
+          const double2 pa = cmul(coef[j], load(j, x));
+          b = make_double2(b.x - pa.x, b.y - pa.y);
+        }
+        store(k, x, b);
+      }
+      __syncthreads();
+    }
+    double acc = 0.0;
+    for (int64_t x = tid; x < len; x += blockDim.x) acc += cabs2(load(k, x));
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) s_val[wave] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int q = 0; q < nwaves; ++q) t += s_val[q];
+      s_nrm = sqrt(t);
+    }
+    __syncthreads();
+    const double nrm = s_nrm;
+    for (int64_t x = tid; x < len; x += blockDim.x) {
+      const double2 b = load(k, x);
+      store(k, x, make_double2(b.x / nrm, b.y / nrm));
+    }
+    __syncthreads();
   }
 }
 
@@ -777,9 +854,11 @@ unsigned blocks_for(int64_t n) {
 
 }  // namespace
 
+// Enqueue the SVD.  With kmax_dev == nullptr every slot is completed; otherwise
+// svd_complete_enqueue must follow, limited to the slots the caller keeps.
 int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
                 double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
-                int* info_dev_off) {
+                int* info_dev_off, bool complete = true) {
   DeviceInfo di;
   int st = device_info(&di);
   if (st) return st;
@@ -810,8 +889,13 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   const double tol = 8.0 * sqrt((double)s.len) * 2.220446049250313e-16;
   int max_sweeps = MAX_SWEEPS;
   void* args[] = {(void*)&s, (void*)&w, (void*)&tol, (void*)&max_sweeps};
-  SEEMPS_HIP(hipLaunchCooperativeKernel((const void*)svd_jacobi, dim3(grid), dim3(256), args, 0,
-                                        stream));
+  {
+    // nominal SVD flops (complex): 4 * (6 M N^2 + 16 N^3), M >= N
+    const double M_ = (double)s.lenA, N_ = (double)s.ncol;
+    TimedLaunch timed("svd_jacobi", stream, 4.0 * (6.0 * M_ * N_ * N_ + 16.0 * N_ * N_ * N_));
+    SEEMPS_HIP(hipLaunchCooperativeKernel((const void*)svd_jacobi, dim3(grid), dim3(256), args, 0,
+                                          stream));
+  }
   hipLaunchKernelGGL(svd_sigma, dim3((unsigned)(s.ncolp < 1024 ? s.ncolp : 1024)), dim3(256), 0,
                      stream, s, w);
   if ((st = check_launch("svd_sigma"))) return st;
@@ -826,10 +910,27 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   const int64_t big = s.ncol * (s.len > s.lenA ? s.len : s.lenA);
   hipLaunchKernelGGL(svd_output, dim3(blocks_for(big)), dim3(256), 0, stream, s, w, U, S, VH);
   if ((st = check_launch("svd_output"))) return st;
-  hipLaunchKernelGGL(svd_complete, dim3(1), dim3(256), 0, stream, s, w, U, VH);
-  if ((st = check_launch("svd_complete"))) return st;
+  if (complete) {
+    TimedLaunch timed("svd_complete", stream, 0.0);
+    hipLaunchKernelGGL(svd_complete, dim3(1), dim3(1024), 0, stream, s, w, U, VH,
+                       (const int64_t*)nullptr);
+    if ((st = check_launch("svd_complete"))) return st;
+  }
   if (info_dev_off) *info_dev_off = (int)((char*)w.ctrl - (char*)work);
   return SEEMPS_OK;
+}
+
+// Orthonormal completion of the slots k < *kmax_dev (see svd_complete).
+int svd_complete_enqueue(int64_t m, int64_t n, double2* U, double2* VH, void* work,
+                         const int64_t* kmax_dev, hipStream_t stream) {
+  DeviceInfo di;
+  int st = device_info(&di);
+  if (st) return st;
+  const SvdShape s = svd_shape(m, n, di.num_cu);
+  SvdWork w = svd_carve(s, work);
+  TimedLaunch timed("svd_complete", stream, 0.0);
+  hipLaunchKernelGGL(svd_complete, dim3(1), dim3(1024), 0, stream, s, w, U, VH, kmax_dev);
+  return check_launch("svd_complete");
 }
 
 int64_t svd_workspace_bytes(int64_t m, int64_t n) {
@@ -851,12 +952,7 @@ int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t
 
 // Read the control block after the stream has been synchronised by the caller's copy.
 int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream) {
-  struct {
-    unsigned int barrier, abort;
-    unsigned long long off[2];
-    int info[4];
-    long long prof[8];
-  } h;
+  Ctrl h;
   SEEMPS_HIP(hipMemcpyAsync(&h, (char*)work + off, sizeof(h), hipMemcpyDeviceToHost, stream));
   SEEMPS_HIP(hipStreamSynchronize(stream));
   if (info) {
@@ -879,7 +975,12 @@ int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream) {
     return SEEMPS_ERR_RUNTIME;
   }
   if (!h.info[1]) {
-    set_error("SVD did not converge in %d sweeps", h.info[0]);
+    const int ns = h.info[0] < 64 ? h.info[0] : 64;
+    set_error("SVD did not converge in %d sweeps (off-diagonal measure by sweep: %.2e %.2e %.2e "
+              "... %.2e %.2e; negligible column norm^2 %.2e)",
+              h.info[0], ns > 0 ? h.offhist[0] : 0.0, ns > 1 ? h.offhist[1] : 0.0,
+              ns > 2 ? h.offhist[2] : 0.0, ns > 1 ? h.offhist[ns - 2] : 0.0,
+              ns > 0 ? h.offhist[ns - 1] : 0.0, h.tiny2);
     return SEEMPS_ERR_NOCONVERGE;
   }
   return SEEMPS_OK;
@@ -938,10 +1039,11 @@ extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, i
   p += al(64);
   int off = 0;
   int st = seemps::svd_enqueue(m, n, (const double2*)A, n, U, S, VH, p,
-                               work_bytes - (p - (char*)work), st_, &off);
+                               work_bytes - (p - (char*)work), st_, &off, false);
   if (st) return st;
   if ((st = seemps::truncate_enqueue(r, S, method, tolerance, max_bond, Ddev, edev, st_)))
     return st;
+  if ((st = seemps::svd_complete_enqueue(m, n, U, VH, p, Ddev, st_))) return st;
   int64_t hD = 0;
   double herr = 0.0;
   SEEMPS_HIP(hipMemcpyAsync(&hD, Ddev, 8, hipMemcpyDeviceToHost, st_));

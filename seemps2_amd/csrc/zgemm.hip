@@ -227,10 +227,12 @@ __global__ void splitk_reduce(int64_t M, int64_t N, int nz, const double2* __res
   }
 }
 
+// flops: algorithmic count of the whole product (for kernel timing).
 int dispatch(int opA, int opB, dim3 grid, hipStream_t stream, int64_t M, int64_t N, int64_t K,
              int64_t Klast, double2 alpha, const double2* a, int64_t lda, int64_t sA,
              const double2* b, int64_t ldb, int64_t sB, double2 beta, double2* c, int64_t ldc,
-             int64_t sC) {
+             int64_t sC, double flops) {
+  TimedLaunch timed("zgemm_kernel", stream, flops);
 #define SEEMPS_GEMM_CASE(x, y)                                                               \
   if (opA == x && opB == y) {                                                                \
     launch<x, y>(grid, stream, M, N, K, Klast, alpha, a, lda, sA, b, ldb, sB, beta, c, ldc, \
@@ -280,10 +282,11 @@ int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, c
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)nz);
   int st = dispatch(opA, opB, grid, stream, M, N, Kc, Klast, make_double2(1.0, 0.0),
                     (const double2*)A, lda, sA, (const double2*)B, ldb, sB,
-                    make_double2(0.0, 0.0), ws, N, M * N);
+                    make_double2(0.0, 0.0), ws, N, M * N, 8.0 * M * N * K);
   if (st) return st;
   int64_t blocks = ceil_div(M * N, 256);
   if (blocks > 4096) blocks = 4096;
+  TimedLaunch timed("splitk_reduce", stream, 0.0);
   hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)blocks), dim3(256), 0, stream, M, N, (int)nz,
                      ws, alpha, beta, (double2*)C, ldc);
   return check_launch("splitk_reduce");
@@ -306,7 +309,8 @@ int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, cons
   }
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)batch);
   return dispatch(opA, opB, grid, stream, M, N, K, K, alpha, (const double2*)A, lda, sA,
-                  (const double2*)B, ldb, sB, beta, (double2*)C, ldc, sC);
+                  (const double2*)B, ldb, sB, beta, (double2*)C, ldc, sC,
+                  8.0 * M * N * K * batch);
 }
 
 }  // namespace seemps

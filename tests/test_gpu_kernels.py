@@ -117,6 +117,20 @@ def test_svd_rank_deficient_and_zero(K, rng):
         assert np.abs(VH @ VH.conj().T - np.eye(r)).max() < 1e-12
 
 
+def test_svd_split_rank_deficient_keeps_isometry(K, rng):
+    """Kept columns for zero singular values (no truncation) still form an isometry."""
+    for (a, d, b), rank in [((16, 2, 16), 3), ((40, 2, 24), 5), ((4, 2, 4), 1)]:
+        AA = (crandn(rng, a * d, rank) @ crandn(rng, rank, d * b)).reshape(a, d, d, b)
+        for mode in (0, 1):
+            Ad, Bd, D, err2, _ = K.svd_split(K.as_dev(AA), a * d, d * b, mode, 0, 0.0, 10 ** 6)
+            Ad, Bd = K.to_numpy(Ad), K.to_numpy(Bd)
+            assert D == min(a * d, d * b)
+            assert err2 == 0.0
+            assert rel(Ad @ Bd, AA.reshape(a * d, d * b)) < 1e-12
+            iso = Ad.conj().T @ Ad if mode == 0 else Bd @ Bd.conj().T
+            assert np.abs(iso - np.eye(D)).max() < 1e-12
+
+
 def test_svd_split_matches_oracle(K, rng):
     for (a, d, b), tol, mb in [((8, 2, 8), 1e-3, 100), ((30, 2, 30), 1e-10, 20), ((1, 2, 1), 1e-16, 5)]:
         AA = crandn(rng, a, d, d, b)
