@@ -169,7 +169,8 @@ int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, const void* 
                     void* out, void* stream);
 
 /* Lowest eigenpair of the Hermitian two-site operator above for bra = ket
- * (a = b = chiL, g = f = chiR), Lanczos with full re-orthogonalisation, Krylov dimension ncv, explicit
+ * (a = b = chiL, g = f = chiR), Lanczos with full re-orthogonalisation (classical
+ * Gram-Schmidt with the DGKS correction), Krylov dimension ncv (2..23), explicit
  * restarts from the Ritz vector, started from v0; stops when the Ritz residual
  * |beta_m y_m| <= tol * max(eps^(2/3), |theta|) (ARPACK's test; tol = 0 means
  * machine epsilon).  The Krylov recurrence, the tridiagonal eigensolve (FP64

@@ -153,7 +153,8 @@ class AntilinearForm:
 
     def tensor1site(self):
         c = self.center
-        return np.einsum("li,ijk,kn->ljn", self.L[c], self.ket.tensors[c], self.R[c])
+        return np.einsum("li,ijk,kn->ljn", self.L[c], self.ket.tensors[c], self.R[c],
+                         optimize=True)
 
     def tensor2site(self, direction):
         if direction > 0:
@@ -162,7 +163,7 @@ class AntilinearForm:
             i, j = self.center - 1, self.center
         return np.einsum(
             "li,ijk,kmo,op->ljmp",
-            self.L[i], self.ket.tensors[i], self.ket.tensors[j], self.R[j],
+            self.L[i], self.ket.tensors[i], self.ket.tensors[j], self.R[j], optimize=True,
         )
 
     def update_right(self):
@@ -340,13 +341,19 @@ def mpo_apply(mpo: list, state, strategy: Strategy, simplify_flag=None):
 
 
 def update_left_mpo_environment(rho, A, O, B):
-    """``environments.py:19-30``: acb,ajd,cjie,bif->def (A = bra)."""
-    return np.einsum("acb,ajd,cjie,bif->def", rho, A.conj(), O, B)
+    """``environments.py:19-30``: acb,ajd,cjie,bif->def (A = bra), as three
+    pairwise contractions (bra, MPO, ket), the order the reference uses."""
+    t = np.tensordot(rho, A.conj(), axes=([0], [0]))  # c b j d
+    t = np.tensordot(t, O, axes=([0, 2], [0, 1]))  # b d i e
+    return np.tensordot(t, B, axes=([0, 2], [0, 1]))  # d e f
 
 
 def update_right_mpo_environment(rho, A, O, B):
-    """``environments.py:33-42``: def,ajd,cjie,bif->acb (A = bra)."""
-    return np.einsum("def,ajd,cjie,bif->acb", rho, A.conj(), O, B)
+    """``environments.py:33-42``: def,ajd,cjie,bif->acb (A = bra), as three
+    pairwise contractions (bra, MPO, ket)."""
+    t = np.tensordot(A.conj(), rho, axes=([2], [0]))  # a j e f
+    t = np.tensordot(t, O, axes=([1, 2], [1, 3]))  # a f c i
+    return np.tensordot(t, B, axes=([1, 3], [2, 1]))  # a c b
 
 
 def mpo_expectation(mpo: list, bra: OMPS, ket: OMPS | None = None):

@@ -187,7 +187,7 @@ def contract_nrjl_ijk_klm(U: np.ndarray, A: np.ndarray, B: np.ndarray):
     a, d, b = A.shape
     _, e, c = B.shape
     U4 = U.reshape(d, e, d, e)
-    return np.einsum("ijk,klm,nrjl->inrm", A, B, U4)
+    return np.einsum("ijk,klm,nrjl->inrm", A, B, U4, optimize=True)
 
 
 def update_in_canonical_form_right(state: list, A: np.ndarray, site: int, strategy):
@@ -240,12 +240,12 @@ def recanonicalize(state: list, oldcenter: int, newcenter: int, strategy) -> flo
 
 def update_left_environment(B: np.ndarray, A: np.ndarray, rho: np.ndarray):
     """``environments.pxi:10-32``: rho'[n,k] = sum B*[l,j,n] rho[l,i] A[i,j,k]."""
-    return np.einsum("ljn,li,ijk->nk", B.conj(), rho, A)
+    return np.einsum("ljn,li,ijk->nk", B.conj(), rho, A, optimize=True)
 
 
 def update_right_environment(B: np.ndarray, A: np.ndarray, rho: np.ndarray):
     """``environments.pxi:39-62``: rho'[i,l] = sum A[i,j,k] rho[k,n] B*[l,j,n]."""
-    return np.einsum("ijk,kn,ljn->il", A, rho, B.conj())
+    return np.einsum("ijk,kn,ljn->il", A, rho, B.conj(), optimize=True)
 
 
 def join_environments(rhoL: np.ndarray, rhoR: np.ndarray):

@@ -52,23 +52,6 @@ __global__ void permute_kernel(int64_t total, PermDesc p, const double2* __restr
   }
 }
 
-// w[i] -= sum_{j<k} V[j*ldv + i] * h[j]
-__global__ void maxpy_kernel(int64_t n, int k, const double2* __restrict__ V, int64_t ldv,
-                             const double2* __restrict__ h, double2* __restrict__ w) {
-  __shared__ double2 hs[64];
-  if (threadIdx.x < k) hs[threadIdx.x] = h[threadIdx.x];
-  __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double2 acc = w[i];
-    for (int j = 0; j < k; ++j) {
-      const double2 t = cmul(V[j * ldv + i], hs[j]);
-      acc.x -= t.x;
-      acc.y -= t.y;
-    }
-    w[i] = acc;
-  }
-}
 
 // x[i] = sum_{j<k} V[j*ldv + i] * y[j]  (y real)
 __global__ void combine_kernel(int64_t n, int k, const double2* __restrict__ V, int64_t ldv,
@@ -123,65 +106,177 @@ __device__ double2 lz_block_sum(double2 v, double2* red) {
   return t;
 }
 
-// part[b * LZ_NV + j] = sum over chunk b of conj(V_j) w, j < k.
-__global__ __launch_bounds__(256) void lz_dots(int64_t n, int k, const double2* __restrict__ V,
-                                               const double2* __restrict__ w,
-                                               double2* __restrict__ part) {
-  __shared__ double2 red[4];
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-  for (int j = 0; j < k; ++j) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc = cfmac(V[j * n + i], w[i], acc);
-    const double2 t = lz_block_sum(acc, red);
-    if (threadIdx.x == 0) part[blockIdx.x * LZ_NV + j] = t;
+// Classical Gram-Schmidt of w against V[0..k) with the DGKS second pass
+// (applied only when the first pass removed more than half of |w|^2), in three
+// launches over lz_blocks(n) chunks.  Each block keeps the k entries V[j][i] of
+// its elements in registers (k <= LZ_KREG), so every launch reads V once.
+// Partial sums are reduced redundantly by every block in a fixed order, which
+// keeps the result deterministic without an extra launch.
+constexpr int LZ_KREG = 24;
+
+// Number of chunks of the orthogonalisation launches.
+inline int lz_blocks(int64_t n) {
+  int64_t b = ceil_div(n, 1024);
+  if (b > 256) b = 256;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// Sum over blocks of per-thread partials (k complex values and one real per
+// thread) into part[blockIdx.x * LZ_NV + j] and pn[blockIdx.x].
+__device__ void lz_store_partials(const double2* acc, int k, double nrm, double2* red,
+                                  double* redn, double2* __restrict__ part,
+                                  double* __restrict__ pn) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < LZ_KREG; ++j) {
+    if (j < k) {
+      double2 v = acc[j];
+      for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_down(v.x, off, 64);
+        v.y += __shfl_down(v.y, off, 64);
+      }
+      if (lane == 0) red[wave * LZ_KREG + j] = v;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) nrm += __shfl_down(nrm, off, 64);
+  if (lane == 0) redn[wave] = nrm;
+  __syncthreads();
+  const int nw = (int)(blockDim.x >> 6);
+  if ((int)threadIdx.x < k) {
+    double2 t = make_double2(0.0, 0.0);
+    for (int q = 0; q < nw; ++q) t = cadd(t, red[q * LZ_KREG + threadIdx.x]);
+    part[blockIdx.x * LZ_NV + threadIdx.x] = t;
+  }
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < nw; ++q) t += redn[q];
+    pn[blockIdx.x] = t;
   }
 }
 
-// h = sum_b part_in (fixed order, in every block); w -= V h over this block's
-// chunk; then either the next dots (part_out, want_norm = 0) or the squared
-// norm (pnorm, want_norm = 1) of the updated chunk.  Block 0 stores h.
-__global__ __launch_bounds__(256) void lz_sub(int64_t n, int k, const double2* __restrict__ V,
-                                              double2* __restrict__ w,
-                                              const double2* __restrict__ part_in,
-                                              double2* __restrict__ part_out,
-                                              double* __restrict__ pnorm, double2* __restrict__ h_out,
-                                              int want_norm) {
-  __shared__ double2 hs[64];
-  __shared__ double2 red[4];
-  const int nb = gridDim.x;
+// h[j] = sum_b part[b * LZ_NV + j] (fixed order) into LDS, j < k.
+__device__ void lz_load_coeffs(int k, int nb, const double2* __restrict__ part, double2* hs) {
   if ((int)threadIdx.x < k) {
     double2 t = make_double2(0.0, 0.0);
-    for (int b = 0; b < nb; ++b) t = cadd(t, part_in[b * LZ_NV + threadIdx.x]);
+    for (int b = 0; b < nb; ++b) t = cadd(t, part[b * LZ_NV + threadIdx.x]);
     hs[threadIdx.x] = t;
-    if (blockIdx.x == 0) h_out[threadIdx.x] = t;
   }
-  __syncthreads();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+}
+
+__device__ double lz_sum_norms(int nb, const double* __restrict__ pn) {
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += pn[b];
+  return t;
+}
+
+// Pass 1: part = chunk sums of conj(V_j) w; pn0 = chunk sums of |w|^2.
+__global__ __launch_bounds__(256) void lz_orth1(int64_t n, int k, const double2* __restrict__ V,
+                                                const double2* __restrict__ w,
+                                                double2* __restrict__ part,
+                                                double* __restrict__ pn0) {
+  __shared__ double2 red[4 * LZ_KREG];
+  __shared__ double redn[4];
+  const int64_t chunk = ceil_div(n, (int64_t)gridDim.x);
   const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double2 acc[LZ_KREG];
+#pragma unroll
+  for (int j = 0; j < LZ_KREG; ++j) acc[j] = make_double2(0.0, 0.0);
+  double nrm = 0.0;
   for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    double2 acc = w[i];
-    for (int j = 0; j < k; ++j) {
-      const double2 t = cmul(V[j * n + i], hs[j]);
-      acc.x -= t.x;
-      acc.y -= t.y;
-    }
-    w[i] = acc;
+    const double2 x = w[i];
+    nrm += cabs2(x);
+#pragma unroll
+    for (int j = 0; j < LZ_KREG; ++j)
+      if (j < k) acc[j] = cfmac(V[j * n + i], x, acc[j]);
   }
-  if (want_norm) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) acc.x += cabs2(w[i]);
-    const double2 t = lz_block_sum(acc, red);
-    if (threadIdx.x == 0) pnorm[blockIdx.x] = t.x;
-  } else {
-    for (int j = 0; j < k; ++j) {
-      double2 acc = make_double2(0.0, 0.0);
-      for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
-        acc = cfmac(V[j * n + i], w[i], acc);
-      const double2 t = lz_block_sum(acc, red);
-      if (threadIdx.x == 0) part_out[blockIdx.x * LZ_NV + j] = t;
+  lz_store_partials(acc, k, nrm, red, redn, part, pn0);
+}
+
+// Pass 2: h = sum of pass-1 partials (block 0 stores it); w -= V h; then the
+// chunk sums of conj(V_j) w (part_out) and |w|^2 (pn1) of the updated w.
+__global__ __launch_bounds__(256) void lz_orth2(int64_t n, int k, const double2* __restrict__ V,
+                                                double2* __restrict__ w,
+                                                const double2* __restrict__ part_in,
+                                                double2* __restrict__ part_out,
+                                                double* __restrict__ pn1,
+                                                double2* __restrict__ h_out) {
+  __shared__ double2 hs[LZ_KREG];
+  __shared__ double2 red[4 * LZ_KREG];
+  __shared__ double redn[4];
+  lz_load_coeffs(k, gridDim.x, part_in, hs);
+  __syncthreads();
+  if (blockIdx.x == 0 && (int)threadIdx.x < k) h_out[threadIdx.x] = hs[threadIdx.x];
+  const int64_t chunk = ceil_div(n, (int64_t)gridDim.x);
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double2 acc[LZ_KREG];
+#pragma unroll
+  for (int j = 0; j < LZ_KREG; ++j) acc[j] = make_double2(0.0, 0.0);
+  double nrm = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    double2 vr[LZ_KREG];
+    double2 x = w[i];
+#pragma unroll
+    for (int j = 0; j < LZ_KREG; ++j) {
+      if (j < k) {
+        vr[j] = V[j * n + i];
+        const double2 t = cmul(vr[j], hs[j]);
+        x.x -= t.x;
+        x.y -= t.y;
+      }
     }
+    w[i] = x;
+    nrm += cabs2(x);
+#pragma unroll
+    for (int j = 0; j < LZ_KREG; ++j)
+      if (j < k) acc[j] = cfmac(vr[j], x, acc[j]);
   }
+  lz_store_partials(acc, k, nrm, red, redn, part_out, pn1);
+}
+
+// Pass 3 (DGKS): if |w|^2 after pass 1 is at least half of |w|^2 before it,
+// the first pass was accurate and w is kept (h2 = 0); otherwise w -= V h2 with
+// h2 = the pass-2 coefficients.  pn2 receives the final chunk norms.
+__global__ __launch_bounds__(256) void lz_orth3(int64_t n, int k, const double2* __restrict__ V,
+                                                double2* __restrict__ w,
+                                                const double2* __restrict__ part_in,
+                                                const double* __restrict__ pn0,
+                                                const double* __restrict__ pn1,
+                                                double* __restrict__ pn2,
+                                                double2* __restrict__ h2_out) {
+  __shared__ double2 hs[LZ_KREG];
+  __shared__ double s_nrm[2];
+  __shared__ double2 red[4];
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) s_nrm[0] = lz_sum_norms(nb, pn0);
+  if (threadIdx.x == 64) s_nrm[1] = lz_sum_norms(nb, pn1);
+  lz_load_coeffs(k, nb, part_in, hs);
+  __syncthreads();
+  const bool second = !(s_nrm[1] >= 0.5 * s_nrm[0]);
+  if (blockIdx.x == 0 && (int)threadIdx.x < k)
+    h2_out[threadIdx.x] = second ? hs[threadIdx.x] : make_double2(0.0, 0.0);
+  if (!second) {
+    if (threadIdx.x == 0) pn2[blockIdx.x] = pn1[blockIdx.x];
+    return;
+  }
+  const int64_t chunk = ceil_div(n, (int64_t)gridDim.x);
+  const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double nrm = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    double2 x = w[i];
+#pragma unroll
+    for (int j = 0; j < LZ_KREG; ++j) {
+      if (j < k) {
+        const double2 t = cmul(V[j * n + i], hs[j]);
+        x.x -= t.x;
+        x.y -= t.y;
+      }
+    }
+    w[i] = x;
+    nrm += cabs2(x);
+  }
+  const double2 t = lz_block_sum(make_double2(nrm, 0.0), red);
+  if (threadIdx.x == 0) pn2[blockIdx.x] = t.x;
 }
 
 // Step j done: beta_j = ||w|| (from the partial norms, in every block),
@@ -219,38 +314,6 @@ __global__ __launch_bounds__(256) void lz_finish(int64_t n, int nparts, int j,
        i += (int64_t)gridDim.x * blockDim.x) {
     const double2 x = w[i];
     w[i] = make_double2(x.x * inv, x.y * inv);
-  }
-}
-
-// After the two orthogonalisation passes of step j: record alpha_j, beta_j.
-__global__ void lz_record(LzScalars* s, int j) {
-  if (threadIdx.x != 0) return;
-  const double a = s->h[j].x + s->h2[j].x;
-  s->alpha[j] = a;
-  const double b = sqrt(fmax(s->nrm.x, 0.0));
-  const double prev = (j > 0) ? s->beta[j - 1] : 0.0;
-  const double ref = fmax(s->scale, fabs(a) + prev);
-  s->scale = ref;
-  if (s->broken || b <= 1e-12 * ref) {
-    if (!s->broken) {
-      s->broken = 1;
-      s->kd = j + 1;
-    }
-    s->beta[j] = 0.0;
-  } else {
-    s->beta[j] = b;
-  }
-}
-
-// v = w / beta_j (zero vector after a breakdown).
-__global__ void lz_normalize(int64_t n, const LzScalars* __restrict__ s, int j,
-                             const double2* __restrict__ w, double2* __restrict__ v) {
-  const double b = s->beta[j];
-  const double inv = (b > 0.0) ? 1.0 / b : 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double2 x = w[i];
-    v[i] = make_double2(x.x * inv, x.y * inv);
   }
 }
 
@@ -401,12 +464,18 @@ struct EigWork {
   double2* x2;     // a*d1*d2*w2*f
   double2* part;   // reduction partials (64 x nblk)
   double2* part2;  // second-pass partials (64 x nblk)
-  double* pnorm;   // partial squared norms (nblk)
+  double* pnorm;   // partial squared norms (3 x nblk: before pass 1, after it, final)
   double2* ks;     // split-K partials of the matvec GEMMs
   LzScalars* sc;
 };
 
 inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+// Partial-sum slots needed by both the Lanczos passes and zmdotc_impl.
+inline int64_t part_blocks(int64_t n) {
+  const int64_t a = lz_blocks(n), b = reduce_blocks(n);
+  return a > b ? a : b;
+}
 
 // Split-K scratch (elements) of the two matvec GEMMs.
 int64_t matvec_ks_elems(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2, int64_t w2,
@@ -423,9 +492,9 @@ int64_t eig_bytes_full(int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t 
   b += al256((m + 1) * n * 16);
   b += al256(chiL * c * d1 * d2 * chiR * 16);
   b += al256(chiL * d1 * d2 * w2 * chiR * 16);
-  b += al256(64 * reduce_blocks(n) * 16);
-  b += al256(64 * reduce_blocks(n) * 16);
-  b += al256(reduce_blocks(n) * 8);
+  b += al256(64 * part_blocks(n) * 16);
+  b += al256(64 * part_blocks(n) * 16);
+  b += al256(3 * part_blocks(n) * 8);
   b += al256(matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR) * 16);
   b += al256(sizeof(LzScalars));
   return b;
@@ -443,11 +512,11 @@ EigWork eig_carve(void* work, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
   w.x2 = (double2*)p;
   p += al256(chiL * d1 * d2 * w2 * chiR * 16);
   w.part = (double2*)p;
-  p += al256(64 * reduce_blocks(n) * 16);
+  p += al256(64 * part_blocks(n) * 16);
   w.part2 = (double2*)p;
-  p += al256(64 * reduce_blocks(n) * 16);
+  p += al256(64 * part_blocks(n) * 16);
   w.pnorm = (double*)p;
-  p += al256(reduce_blocks(n) * 8);
+  p += al256(3 * part_blocks(n) * 8);
   w.ks = (double2*)p;
   p += al256(matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR) * 16);
   w.sc = (LzScalars*)p;
@@ -559,7 +628,8 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
   hipStream_t st_ = (hipStream_t)stream;
   SEEMPS_CHECK_ARG(chiL >= 1 && chiR >= 1 && c >= 1 && w2 >= 1 && d1 >= 1 && d2 >= 1,
                    "eigensolve: bad dimensions");
-  SEEMPS_CHECK_ARG(ncv >= 2 && ncv <= 63, "eigensolve: ncv must be in [2, 63]");
+  SEEMPS_CHECK_ARG(ncv >= 2 && ncv < LZ_KREG, "eigensolve: ncv must be in [2, %d]",
+                   LZ_KREG - 1);
   SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "eigensolve: w*d1*d2 > 64 not supported");
   SEEMPS_CHECK_ARG(maxiter >= 1, "eigensolve: maxiter must be positive");
   const int64_t n = chiL * d1 * d2 * chiR;
@@ -573,7 +643,7 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
   const double2* H_ = (const double2*)Hm;
   const double2* R_ = (const double2*)R;
   const unsigned gb = grid_for(n);
-  const int nbk = (int)reduce_blocks(n);
+  const int nbk = lz_blocks(n);
   int st;
   // V[0] = v0 / ||v0||
   SEEMPS_HIP(hipMemcpyAsync(w.V, v0, n * 16, hipMemcpyDeviceToDevice, st_));
@@ -594,27 +664,30 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
                                      st_)))
         return st;
       ++matvecs;
-      // Two classical Gram-Schmidt passes against V[0..j], then the norm:
-      // four launches (dots | subtract + dots | subtract + norm | normalise).
+      // Gram-Schmidt against V[0..j] with the DGKS correction, then the norm:
+      // four launches (dots | subtract + dots | conditional subtract | normalise).
       const int kk = (int)(j + 1);
       const double orth = 8.0 * n * kk;
+      double* pn0 = w.pnorm;
+      double* pn1 = w.pnorm + nbk;
+      double* pn2 = w.pnorm + 2 * nbk;
       {
-        TimedLaunch timed("lz_dots", st_, orth);
-        hipLaunchKernelGGL(lz_dots, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part);
+        TimedLaunch timed("lz_orth1", st_, orth);
+        hipLaunchKernelGGL(lz_orth1, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part, pn0);
       }
       {
-        TimedLaunch timed("lz_sub", st_, 2.0 * orth);
-        hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part, w.part2,
-                           w.pnorm, w.sc->h, 0);
+        TimedLaunch timed("lz_orth2", st_, 2.0 * orth);
+        hipLaunchKernelGGL(lz_orth2, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part,
+                           w.part2, pn1, w.sc->h);
       }
       {
-        TimedLaunch timed("lz_sub", st_, 2.0 * orth);
-        hipLaunchKernelGGL(lz_sub, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part2, w.part,
-                           w.pnorm, w.sc->h2, 1);
+        TimedLaunch timed("lz_orth3", st_, orth);
+        hipLaunchKernelGGL(lz_orth3, dim3(nbk), dim3(256), 0, st_, n, kk, w.V, wv, w.part2, pn0,
+                           pn1, pn2, w.sc->h2);
       }
       {
         TimedLaunch timed("lz_finish", st_, 0.0);
-        hipLaunchKernelGGL(lz_finish, dim3(gb), dim3(256), 0, st_, n, nbk, (int)j, w.pnorm, w.sc,
+        hipLaunchKernelGGL(lz_finish, dim3(gb), dim3(256), 0, st_, n, nbk, (int)j, pn2, w.sc,
                            wv);
       }
       if ((st = check_launch("lanczos step"))) return st;

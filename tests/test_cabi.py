@@ -46,6 +46,27 @@ def test_host_only_entry_points():
     assert lib.seemps_eigensolve_workspace(256, 5, 2, 2, 5, 256, 20) > 21 * 262144 * 16
 
 
+@pytest.mark.skipif(not LIB.exists(), reason="libseemps_hip.so not built")
+def test_timing_registry_host_side():
+    """Enabling, reading and disabling the kernel timers needs no device when nothing ran."""
+    lib = ctypes.CDLL(str(LIB))
+    lib.seemps_timing_enable.argtypes = [ctypes.c_char_p]
+    lib.seemps_timing_get.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+    lib.seemps_last_error.restype = ctypes.c_char_p
+    assert lib.seemps_timing_enable(b"svd_jacobi,zgemm_kernel") == 0
+    assert lib.seemps_timing_count() == 0
+    n, ms, fl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    buf = ctypes.create_string_buffer(32)
+    assert lib.seemps_timing_get(0, buf, 32, ctypes.byref(n), ctypes.byref(ms),
+                                 ctypes.byref(fl)) == 1  # SEEMPS_ERR_INVALID
+    assert b"out of range" in lib.seemps_last_error()
+    assert lib.seemps_timing_enable(None) == 0
+    assert lib.seemps_timing_count() == 0
+
+
 def test_product_package_does_not_import_oracle():
     for py in (ROOT / "seemps2_amd").rglob("*.py"):
         src = py.read_text()
