@@ -59,6 +59,14 @@ def parse():
     return ap.parse_args()
 
 
+def kernel_base_name(name):
+    """'void ns::(anonymous namespace)::kern<1, 0>(args)' -> 'kern'."""
+    name = name.replace("(anonymous namespace)::", "")
+    if name.startswith("void "):
+        name = name[5:]
+    return name.split("(")[0].split("<")[0].split("::")[-1]
+
+
 def rocprof_summary(directory, bench_json=None, top=25):
     """Markdown table from rocprofv3 --kernel-trace --stats CSV files."""
     import csv
@@ -72,9 +80,10 @@ def rocprof_summary(directory, bench_json=None, top=25):
     out = ["| kernel | calls | total ms | avg us | min us | max us | % |",
            "|---|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
-        name = r["Name"].replace("|", "/")
-        if len(name) > 90:
-            name = name[:87] + "..."
+        name = kernel_base_name(r["Name"])
+        targs = r["Name"].replace("(anonymous namespace)", "").split("(")[0]
+        if "<" in targs:
+            name += "<" + targs.split("<", 1)[1]
         out.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
                    f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
                    f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
@@ -85,9 +94,7 @@ def rocprof_summary(directory, bench_json=None, top=25):
         if kern and nlaunch:
             durs = []
             for r in csv.DictReader(open(trace[0])):
-                name = r.get("Kernel_Name", "")
-                base = name.split("(")[0].split("<")[0].split("::")[-1]
-                if base == kern:
+                if kernel_base_name(r.get("Kernel_Name", "")) == kern:
                     durs.append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
             last = durs[-nlaunch:]
             if last:
