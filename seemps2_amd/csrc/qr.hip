@@ -267,9 +267,10 @@ __global__ __launch_bounds__(QR_LDS_THREADS) void qr_panel_lds_kernel(
 }  // namespace
 
 // In-place blocked QR of A_cm (ncol columns of length len, len >= ncol).
-// Work: Vall (ncol * len complex), Tall (npanels * NB * NB), tau (ncol), Z (NB * ncol * 2).
+// Work: Vall (ncol * len complex), Tall (npanels * NB * NB), tau (ncol), Z (NB * ncol * 2),
+// ks (ks_elems complex, split-K partials of the NB-row products; may be 0).
 int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
-              double2* tau, double2* Z, hipStream_t stream) {
+              double2* tau, double2* Z, double2* ks, int64_t ks_elems, hipStream_t stream) {
   int st;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0),
                 mone = make_double2(-1.0, 0.0);
@@ -295,8 +296,8 @@ int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, 
     double2* Z1 = Z;                          // nb x ntr
     double2* Z2 = Z + nb_max * ncol;          // nb x ntr
     // Z1 = conj(Vp) At^T ; Z2 = T^H Z1 ; At -= Z2^T Vp
-    if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_T, nb, ntr, L, one, Vp, L, 0, At, len, 0, zero, Z1, ntr,
-                    0, 1, stream)))
+    if ((st = zgemm_ws(SEEMPS_OP_C, SEEMPS_OP_T, nb, ntr, L, one, Vp, L, At, len, zero, Z1, ntr,
+                       ks, ks_elems, stream)))
       return st;
     if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_N, nb, ntr, nb, one, T, nb, 0, Z1, ntr, 0, zero, Z2, ntr,
                     0, 1, stream)))
@@ -310,7 +311,8 @@ int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, 
 
 // Y <- Q Y for Y column-major (ny columns of length len), Q from qr_factor.
 int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,
-               const double2* Tall, double2* Z, hipStream_t stream) {
+               const double2* Tall, double2* Z, double2* ks, int64_t ks_elems,
+               hipStream_t stream) {
   int st;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0),
                 mone = make_double2(-1.0, 0.0);
@@ -324,8 +326,8 @@ int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, co
     double2* Yt = Y + j0;  // ny x L, row stride len
     double2* Z1 = Z;
     double2* Z2 = Z + nb_max * (ny > ncol ? ny : ncol);
-    if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_T, nb, ny, L, one, Vp, L, 0, Yt, len, 0, zero, Z1, ny, 0,
-                    1, stream)))
+    if ((st = zgemm_ws(SEEMPS_OP_C, SEEMPS_OP_T, nb, ny, L, one, Vp, L, Yt, len, zero, Z1, ny, ks,
+                       ks_elems, stream)))
       return st;
     if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, nb, ny, nb, one, T, nb, 0, Z1, ny, 0, zero, Z2, ny, 0,
                     1, stream)))

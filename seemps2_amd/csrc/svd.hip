@@ -33,9 +33,10 @@
 namespace seemps {
 
 int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
-              double2* tau, double2* Z, hipStream_t stream);
+              double2* tau, double2* Z, double2* ks, int64_t ks_elems, hipStream_t stream);
 int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,
-               const double2* Tall, double2* Z, hipStream_t stream);
+               const double2* Tall, double2* Z, double2* ks, int64_t ks_elems,
+               hipStream_t stream);
 
 namespace {
 
@@ -78,10 +79,13 @@ struct SvdWork {
   double2* tau;   // ncol
   double2* Z;     // 2 x QRNB x max(ncol, ncolp)
   double2* Y;     // ncol x lenA (Q V)
+  double2* ks;    // split-K partials of the QR panel products
   double2* W;     // ncolp x len
   double2* V;     // ncolp x ncolp
   double2* part;  // 2 x P x RMAX x PW x PW (partial Grams, by step parity)
   unsigned int* sync;  // P pair arrival counters, then P x R chunk step counters
+  double* colnrm;  // ncol squared column norms of B (QR path)
+  int* colperm;    // ncol: B's columns by decreasing norm (QR path)
   double* sigma;  // ncolp
   int* perm;      // ncolp
   Ctrl* ctrl;
@@ -120,6 +124,10 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
   return s;
 }
 
+// Split-K scratch of the NB-row products of the QR (factor and apply).
+// (Up to 8 slices of NB x ncol, enough for every panel's shape.)
+inline int64_t qr_ks_elems(const SvdShape& s) { return 8 * QRNB * s.ncol; }
+
 #define SEEMPS_SVD_REGIONS(X)                                         \
   X(Acm, s.use_qr ? s.ncol * s.lenA : 0, double2)                     \
   X(Vall, s.use_qr ? s.ncol * s.lenA : 0, double2)                    \
@@ -127,10 +135,13 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
   X(tau, s.use_qr ? s.ncol : 0, double2)                              \
   X(Z, s.use_qr ? 2 * QRNB * (s.ncolp > s.lenA ? s.ncolp : s.lenA) : 0, double2) \
   X(Y, s.use_qr ? s.ncol * s.lenA : 0, double2)                       \
+  X(ks, s.use_qr ? qr_ks_elems(s) : 0, double2)                       \
   X(W, s.ncolp * s.len, double2)                                      \
   X(V, s.ncolp * s.ncolp, double2)                                    \
   X(part, 2 * s.P * RMAX * PW * PW, double2)                          \
   X(sync, s.P + s.P * s.R, unsigned int)                              \
+  X(colnrm, s.use_qr ? s.ncol : 0, double)                            \
+  X(colperm, s.use_qr ? s.ncol : 0, int)                              \
   X(sigma, s.ncolp, double)                                           \
   X(perm, s.ncolp, int)                                               \
   X(ctrl, 1, Ctrl)
@@ -157,21 +168,55 @@ SvdWork svd_carve(const SvdShape& s, void* work) {
 // B column-major: dst[c * lenA + r] = A[r][c] (m >= n) or conj(A[c][r]) (m < n),
 // zero for padded columns c >= ncol (dst has ncols_dst columns).
 __global__ void load_colmajor(SvdShape s, const double2* __restrict__ A, int64_t lda,
-                              double2* __restrict__ dst, int64_t ncols_dst) {
+                              double2* __restrict__ dst, int64_t ncols_dst,
+                              const int* __restrict__ perm) {
   const int64_t total = ncols_dst * s.lenA;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = e / s.lenA, r = e % s.lenA;
     double2 v = make_double2(0.0, 0.0);
     if (c < s.ncol) {
+      const int64_t cs = perm ? perm[c] : c;  // column of B stored at position c
       if (!s.transposed) {
-        v = A[r * lda + c];
+        v = A[r * lda + cs];
       } else {
-        v = A[c * lda + r];
+        v = A[cs * lda + r];
         v.y = -v.y;
       }
     }
     dst[e] = v;
+  }
+}
+
+// Squared norms of the ncol columns of B (B = A, or A^H when transposed).
+__global__ void col_norms(SvdShape s, const double2* __restrict__ A, int64_t lda,
+                          double* __restrict__ nrm) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncol;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    if (!s.transposed) {
+      for (int64_t r = 0; r < s.lenA; ++r) acc += cabs2(A[r * lda + c]);
+    } else {
+      for (int64_t r = 0; r < s.lenA; ++r) acc += cabs2(A[c * lda + r]);
+    }
+    nrm[c] = acc;
+  }
+}
+
+// perm[rank] = c with decreasing norm, ties by index (the QR then sees the
+// heaviest columns first, which leaves R^H graded the way one-sided Jacobi
+// converges fast on).
+__global__ void col_rank(SvdShape s, const double* __restrict__ nrm, int* __restrict__ perm) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncol;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    // NaN keys sort last, so the ranks always form a permutation
+    const double nc = isnan(nrm[c]) ? -1.0 : nrm[c];
+    int rank = 0;
+    for (int64_t d = 0; d < s.ncol; ++d) {
+      const double nd = isnan(nrm[d]) ? -1.0 : nrm[d];
+      rank += (nd > nc || (nd == nc && d < c)) ? 1 : 0;
+    }
+    perm[rank] = (int)c;
   }
 }
 
@@ -450,26 +495,37 @@ __device__ double2* inner_jacobi(double2* G, double2* G2, double2* Q, double2* r
   return cur;
 }
 
-// Rotate rows [r0, r1) of the PW columns (bi, bj) of a column-major matrix M
-// (column stride ld) by Q: M_pq <- M_pq Q.  Uses an RT-row LDS tile.
-__device__ void rotate_rows(double2* __restrict__ M, int64_t ld, int64_t bi, int64_t bj,
-                            int64_t r0, int64_t r1, const double2* Q, double2* tile) {
+// Rotate rows [w0, w1) of the pair's PW columns of W (column stride ldw) and
+// rows [v0, v1) of those of V (stride ldv) by Q: M_pq <- M_pq Q.  Both RT-row
+// tiles are staged in LDS together (tw, tv), so one load latency covers both.
+__device__ void rotate_rows(double2* __restrict__ W, int64_t ldw, int64_t w0, int64_t w1,
+                            double2* __restrict__ V, int64_t ldv, int64_t v0, int64_t v1,
+                            int64_t bi, int64_t bj, const double2* Q, double2* tw,
+                            double2* tv) {
   const int tid = threadIdx.x;
-  for (int64_t rb = r0; rb < r1; rb += RT) {
-    const int nr = (int)min((int64_t)RT, r1 - rb);
+  const int64_t nw = ceil_div(w1 - w0, (int64_t)RT), nv = ceil_div(v1 - v0, (int64_t)RT);
+  const int64_t nblk = nw > nv ? nw : nv;
+  for (int64_t blk = 0; blk < nblk; ++blk) {
+    const int64_t rbw = w0 + blk * RT, rbv = v0 + blk * RT;
+    const int nrw = (blk < nw) ? (int)min((int64_t)RT, w1 - rbw) : 0;
+    const int nrv = (blk < nv) ? (int)min((int64_t)RT, v1 - rbv) : 0;
     for (int e = tid; e < PW * RT; e += blockDim.x) {
       const int t = e / RT, rr = e % RT;
-      tile[t * RTP + rr] = (rr < nr) ? M[pair_col(bi, bj, t) * ld + rb + rr] : make_double2(0, 0);
+      tw[t * RTP + rr] = (rr < nrw) ? W[pair_col(bi, bj, t) * ldw + rbw + rr] : make_double2(0, 0);
+      tv[t * RTP + rr] = (rr < nrv) ? V[pair_col(bi, bj, t) * ldv + rbv + rr] : make_double2(0, 0);
     }
     __syncthreads();
     for (int e = tid; e < PW * RT; e += blockDim.x) {
       const int j = e / RT, rr = e % RT;
-      if (rr < nr) {
-        double2 acc = make_double2(0.0, 0.0);
+      double2 aw = make_double2(0.0, 0.0), av = make_double2(0.0, 0.0);
 #pragma unroll 8
-        for (int k = 0; k < PW; ++k) acc = cfma(tile[k * RTP + rr], Q[k * GP + j], acc);
-        M[pair_col(bi, bj, j) * ld + rb + rr] = acc;
+      for (int k = 0; k < PW; ++k) {
+        const double2 q = Q[k * GP + j];
+        aw = cfma(tw[k * RTP + rr], q, aw);
+        av = cfma(tv[k * RTP + rr], q, av);
       }
+      if (rr < nrw) W[pair_col(bi, bj, j) * ldw + rbw + rr] = aw;
+      if (rr < nrv) V[pair_col(bi, bj, j) * ldv + rbv + rr] = av;
     }
     __syncthreads();
   }
@@ -526,16 +582,21 @@ __device__ void pair_update(const SvdShape& s, const SvdWork& w, int64_t bi, int
   const int tid = threadIdx.x;
   const int64_t rc = ceil_div(s.len, s.R);
   const int64_t rcv = ceil_div(s.ncolp, s.R);
+  // Upper triangle from the partials; the lower one mirrors it, so G is
+  // exactly Hermitian.
   for (int e = tid; e < PW * PW; e += blockDim.x) {
+    const int i = e / PW, j = e % PW;
+    Q[i * GP + j] = make_double2((i == j) ? 1.0 : 0.0, 0.0);
+    if (i > j) continue;
     double2 gq[RMAX];
 #pragma unroll
     for (int q = 0; q < RMAX; ++q) gq[q] = (q < s.R) ? src[q * PW * PW + e] : make_double2(0.0, 0.0);
     double2 g = gq[0];
 #pragma unroll
     for (int q = 1; q < RMAX; ++q) g = cadd(g, gq[q]);
-    const int i = e / PW, j = e % PW;
+    if (i == j) g.y = 0.0;
     G[i * GP + j] = g;
-    Q[i * GP + j] = make_double2((i == j) ? 1.0 : 0.0, 0.0);
+    G[j * GP + i] = make_double2(g.x, -g.y);
   }
   __syncthreads();
   // max normalised off-diagonal over non-negligible column pairs (every
@@ -578,8 +639,9 @@ __device__ void pair_update(const SvdShape& s, const SvdWork& w, int64_t bi, int
     pt[3] += t1 - *t0;
     *t0 = t1;
   }
-  rotate_rows(w.W, s.len, bi, bj, r * rc, min(s.len, r * rc + rc), Q, tile);
-  rotate_rows(w.V, s.ncolp, bi, bj, r * rcv, min(s.ncolp, r * rcv + rcv), Q, tile);
+  // G2 is free after the inner solve: it holds the V tile
+  rotate_rows(w.W, s.len, r * rc, min(s.len, r * rc + rc), w.V, s.ncolp, r * rcv,
+              min(s.ncolp, r * rcv + rcv), bi, bj, Q, tile, G2);
   if (prof) {
     const long long t1 = now_ticks();
     pt[4] += t1 - *t0;
@@ -792,10 +854,11 @@ __global__ void svd_sigma(SvdShape s, SvdWork w) {
 __global__ void svd_rank(SvdShape s, SvdWork w) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncolp;
        c += (int64_t)gridDim.x * blockDim.x) {
-    const double sc = w.sigma[c];
+    // NaN keys sort last, so the ranks always form a permutation
+    const double sc = isnan(w.sigma[c]) ? -1.0 : w.sigma[c];
     int rank = 0;
     for (int64_t d = 0; d < s.ncolp; ++d) {
-      const double sd = w.sigma[d];
+      const double sd = isnan(w.sigma[d]) ? -1.0 : w.sigma[d];
       rank += (sd > sc || (sd == sc && d < c)) ? 1 : 0;
     }
     w.perm[rank] = (int)c;
@@ -820,10 +883,12 @@ __global__ void svd_output(SvdShape s, SvdWork w, double2* __restrict__ U, doubl
     double2 v = w.W[c * s.len + x];
     // negligible columns get an orthonormal completion afterwards
     v = (sg > 0.0 && sg * sg > tiny2) ? make_double2(v.x / sg, v.y / sg) : make_double2(0.0, 0.0);
+    // with QR, coordinate x of the W side is B's column colperm[x]
+    const int64_t xo = s.use_qr ? (int64_t)w.colperm[x] : x;
     if (wside_u)
-      U[x * r + k] = v;
+      U[xo * r + k] = v;
     else
-      VH[k * s.n + x] = make_double2(v.x, -v.y);
+      VH[k * s.n + xo] = make_double2(v.x, -v.y);
   }
   const double2* src = s.use_qr ? w.Y : w.V;
   const int64_t ld = s.use_qr ? s.lenA : s.ncolp;
@@ -1061,17 +1126,24 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
                    (long long)work_bytes, (long long)svd_bytes(s));
   SvdWork w = svd_carve(s, work);
   if (s.use_qr) {
+    hipLaunchKernelGGL(col_norms, dim3(blocks_for(s.ncol)), dim3(256), 0, stream, s, A, lda,
+                       w.colnrm);
+    if ((st = check_launch("col_norms"))) return st;
+    hipLaunchKernelGGL(col_rank, dim3(blocks_for(s.ncol)), dim3(256), 0, stream, s, w.colnrm,
+                       w.colperm);
+    if ((st = check_launch("col_rank"))) return st;
     hipLaunchKernelGGL(load_colmajor, dim3(blocks_for(s.ncol * s.lenA)), dim3(256), 0, stream, s,
-                       A, lda, w.Acm, s.ncol);
+                       A, lda, w.Acm, s.ncol, (const int*)w.colperm);
     if ((st = check_launch("load_colmajor"))) return st;
-    if ((st = qr_factor(w.Acm, s.lenA, s.ncol, QRNB, w.Vall, w.Tall, w.tau, w.Z, stream)))
+    if ((st = qr_factor(w.Acm, s.lenA, s.ncol, QRNB, w.Vall, w.Tall, w.tau, w.Z, w.ks,
+                        qr_ks_elems(s), stream)))
       return st;
     hipLaunchKernelGGL(load_w_from_r, dim3(blocks_for(s.ncolp * s.len)), dim3(256), 0, stream, s,
                        w.Acm, w.W);
     if ((st = check_launch("load_w_from_r"))) return st;
   } else {
     hipLaunchKernelGGL(load_colmajor, dim3(blocks_for(s.ncolp * s.lenA)), dim3(256), 0, stream,
-                       s, A, lda, w.W, s.ncolp);
+                       s, A, lda, w.W, s.ncolp, (const int*)nullptr);
     if ((st = check_launch("load_colmajor"))) return st;
   }
   hipLaunchKernelGGL(init_v_ctrl, dim3(blocks_for(s.ncolp * s.ncolp)), dim3(256), 0, stream, s, w);
@@ -1103,7 +1175,8 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   if (s.use_qr) {
     hipLaunchKernelGGL(load_y, dim3(blocks_for(s.ncol * s.lenA)), dim3(256), 0, stream, s, w);
     if ((st = check_launch("load_y"))) return st;
-    if ((st = qr_apply_q(w.Y, s.lenA, s.ncol, s.ncol, QRNB, w.Vall, w.Tall, w.Z, stream)))
+    if ((st = qr_apply_q(w.Y, s.lenA, s.ncol, s.ncol, QRNB, w.Vall, w.Tall, w.Z, w.ks,
+                         qr_ks_elems(s), stream)))
       return st;
   }
   const int64_t big = s.ncol * (s.len > s.lenA ? s.len : s.lenA);
