@@ -187,6 +187,50 @@ int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2, 
                                int64_t maxiter, double* stats, void* work,
                                int64_t work_bytes, void* stream);
 
+/* Fixed-rank randomized variant of seemps_svd_split (no counterpart in the
+ * reference: the learned-rank truncation).  Sketch Y = A Omega with l columns
+ * (Omega complex Gaussian from `seed`), power_iters refinements
+ * Y <- A (A^H Y), Q = orth(Y) (Householder), B = Q^H A, Jacobi SVD of B,
+ * U = Q U_B.  The truncation rule runs on B's l singular values with the
+ * unsketched weight ||A||_F^2 - ||B||_F^2 counted as discarded.  When that
+ * weight exceeds what the rule may discard while max_bond >= l, when a
+ * threshold rule would look past l values, for SEEMPS_TRUNC_NONE, or when
+ * l >= min(m, n), the full
+ * seemps_svd_split result is returned and *fallback_host = 1.  Otherwise same
+ * outputs as seemps_svd_split (s_out receives l values). */
+int64_t seemps_svd_split_sketch_workspace(int64_t m, int64_t n, int64_t l);
+int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A, int mode, int method,
+                            double tolerance, int64_t max_bond, int64_t l, int64_t power_iters,
+                            uint64_t seed, void* outA, void* outB, double* s_out,
+                            int64_t* D_host, double* err_host, int32_t* fallback_host,
+                            void* work, int64_t work_bytes, int32_t* info, void* stream);
+
+/* Features of the learned rank predictor for an m x n matrix A (row-major,
+ * lda = n): 32 sketch columns Y = A Omega (Omega from `seed`), eigenvalues of
+ * Y^H Y, features[0] = log10(min(m, n)) / 4 and features[i] =
+ * clamp(log10(s_i / s_0), -8, 0) / 8 (s = sketch singular values), FP32 on
+ * the device.  No counterpart in the reference. */
+int64_t seemps_spectral_sketch_workspace(int64_t m, int64_t n);
+int seemps_spectral_sketch(int64_t m, int64_t n, const void* A, uint64_t seed, float* features,
+                           void* work, int64_t work_bytes, void* stream);
+
+/* Rank-predictor MLP (FP32, no counterpart in the reference):
+ * y = w3 . relu(W2 relu(W1 x + b1) + b2) + b3 for B samples of F features
+ * with H hidden units; params = W1 (H x F), b1, W2 (H x H), b2, w3 (H), b3.
+ * Forward keeps the activations h1, h2 (B x H) for the backward pass, which
+ * returns the gradients of the mean squared error against `target` (B) in the
+ * same layout as params and the loss itself (device float). */
+int64_t seemps_mlp_param_count(int F, int H);
+int64_t seemps_mlp_scratch_count(int B, int H);
+int seemps_mlp_forward(int B, int F, int H, const float* params, const float* x, float* h1,
+                       float* h2, float* y, void* stream);
+int seemps_mlp_backward(int B, int F, int H, const float* params, const float* x,
+                        const float* h1, const float* h2, const float* y, const float* target,
+                        float* grads, float* loss, float* scratch, void* stream);
+/* Adam update of n parameters (step >= 1, bias-corrected). */
+int seemps_adam_step(int64_t n, float* params, const float* grads, float* m, float* v, float lr,
+                     float beta1, float beta2, float eps, int64_t step, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Kernel timing (no counterpart in the reference: measurement support for
  * benchmarks).  The library brackets each launch of the selected kernels with

@@ -64,6 +64,24 @@ _SIGNATURES = {
         c_int,
         [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _D, _I, POINTER(c_double), _P, _I, _P],
     ),
+    "seemps_svd_split_sketch_workspace": (_I, [_I, _I, _I]),
+    "seemps_svd_split_sketch": (
+        c_int,
+        [_I, _I, _P, c_int, c_int, _D, _I, _I, _I, ctypes.c_uint64, _P, _P, _P,
+         POINTER(c_int64), POINTER(c_double), POINTER(c_int32), _P, _I, POINTER(c_int32), _P],
+    ),
+    "seemps_spectral_sketch_workspace": (_I, [_I, _I]),
+    "seemps_spectral_sketch": (c_int, [_I, _I, _P, ctypes.c_uint64, _P, _P, _I, _P]),
+    "seemps_mlp_param_count": (_I, [c_int, c_int]),
+    "seemps_mlp_scratch_count": (_I, [c_int, c_int]),
+    "seemps_mlp_forward": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
+    "seemps_mlp_backward": (
+        c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    ),
+    "seemps_adam_step": (
+        c_int, [_I, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                ctypes.c_float, _I, _P]
+    ),
     "seemps_timing_enable": (c_int, [ctypes.c_char_p]),
     "seemps_timing_count": (c_int, []),
     "seemps_timing_get": (

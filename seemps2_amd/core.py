@@ -312,10 +312,11 @@ def destructively_truncate_vector(s, strategy: Strategy) -> float:
 
 
 def _split(AA, strategy: Strategy, mode: int):
+    from .learned import two_site_split
+
     AA = K.as_dev(AA)
     a, d1, d2, b = AA.shape
-    A2, B2, D, err2, _ = K.svd_split(AA, a * d1, d2 * b, mode, strategy.method,
-                                     strategy.tolerance, strategy.max_bond_dimension)
+    A2, B2, D, err2 = two_site_split(AA, a * d1, d2 * b, mode, strategy)
     return A2.view(a, d1, D), B2.view(D, d2, b), math.sqrt(err2)
 
 

@@ -111,4 +111,25 @@ int64_t reduce_blocks(int64_t n);
 int zmdotc_impl(int64_t n, int64_t nv, const double2* V, int64_t ldv, const double2* w,
                 double2* result, double2* work, hipStream_t stream);
 
+// SVD and QR building blocks (svd.hip, qr.hip).
+int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
+                double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
+                int* info_dev_off, bool complete = true);
+int svd_complete_enqueue(int64_t m, int64_t n, double2* U, double2* VH, void* work,
+                         const int64_t* kmax_dev, hipStream_t stream);
+int64_t svd_workspace_bytes(int64_t m, int64_t n);
+int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t max_bond,
+                     int64_t* D_out, double* err_out, hipStream_t stream);
+int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream);
+int64_t svd_split_bytes(int64_t m, int64_t n);
+int svd_split_impl(int64_t m, int64_t n, const double2* A, int mode, int method, double tolerance,
+                   int64_t max_bond, double2* outA, double2* outB, double* s_out,
+                   int64_t* D_host, double* err_host, void* work, int64_t work_bytes,
+                   int32_t* info, hipStream_t stream);
+int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
+              double2* tau, double2* Z, double2* ks, int64_t ks_elems, hipStream_t stream);
+int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,
+               const double2* Tall, double2* Z, double2* ks, int64_t ks_elems,
+               hipStream_t stream);
+
 }  // namespace seemps

@@ -1,0 +1,361 @@
+// Fixed-rank randomized SVD split (the learned-rank truncation path).
+//
+// For an m x n matrix A and a sketch size l <= min(m, n):
+//   Y = A Omega (Omega n x l complex Gaussian), optionally refined by power
+//   iterations Y <- A (A^H Y) with re-orthonormalisation, Q = orth(Y)
+//   (Householder), B = Q^H A (l x n), full Jacobi SVD of B, U = Q U_B.
+// The truncation rule runs on the l singular values of B, with the unsketched
+// weight ||A||_F^2 - ||B||_F^2 counted as discarded.  When that weight alone is
+// more than the rule may discard and the bond cap does not bind below l (or a
+// threshold rule would keep more than l values), the result is the full SVD
+// split instead and the fallback flag is set.  Everything stays on the device; the host reads D, the error and the
+// flag once, as seemps_svd_split does.
+#include <cmath>
+
+#include "common.h"
+#include "jacobi_inner.h"
+
+namespace seemps {
+namespace {
+
+inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// out[e] = complex standard Gaussian (Box-Muller on two counter-hash uniforms).
+__global__ void gauss_kernel(int64_t count, uint64_t seed, double2* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h1 = mix64(seed + 0x9e3779b97f4a7c15ull * (2 * (uint64_t)e + 1));
+    const uint64_t h2 = mix64(h1 ^ 0xd1b54a32d192ed03ull);
+    const double u1 = ((h1 >> 11) + 1) * (1.0 / 9007199254740993.0);  // (0, 1]
+    const double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);        // [0, 1)
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586 * u2, &sn, &cs);
+    out[e] = make_double2(r * cs * 0.7071067811865476, r * sn * 0.7071067811865476);
+  }
+}
+
+// Column-major identity block: column c (length len) = e_c, c < ncol.
+__global__ void identity_cm(int64_t len, int64_t ncol, double2* __restrict__ out) {
+  const int64_t total = len * ncol;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / len, r = e % len;
+    out[e] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// Truncation on the sketch spectrum s[0..l) with the unsketched weight
+// frob - sum s^2 counted as discarded (see the file comment).  full = min(m, n).
+__global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __restrict__ s,
+                                       const double2* __restrict__ frob2, int method, double tol,
+                                       int64_t max_bond, int64_t* D_out, double* err_out,
+                                       int* fallback_out) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double cap = 0.0;
+  for (int64_t i = l - 1; i >= 0; --i) cap += s[i] * s[i];
+  const double total = frob2->x;
+  const double tail = (total > cap) ? total - cap : 0.0;
+  int64_t D = l;
+  int fallback = 0;
+  if (method == SEEMPS_TRUNC_NONE) {
+    fallback = 1;
+  } else if (method == SEEMPS_TRUNC_REL_NORM2) {
+    const double max_error = total * tol;
+    if (tail > max_error) {
+      // the tolerance alone would keep more than l values; the sketch still
+      // decides when the bond cap binds below l
+      if (max_bond < l)
+        D = max_bond;
+      else
+        fallback = 1;
+    } else {
+      double acc = tail;
+      int64_t k = 0;
+      for (int64_t i = l - 1; i >= 1; --i) {  // keep at least one value
+        const double x = s[i] * s[i];
+        if (acc + x > max_error) break;
+        acc += x;
+        ++k;
+      }
+      D = l - k;
+    }
+  } else {
+    const double thr = (method == SEEMPS_TRUNC_REL_SV) ? tol * s[0] : tol;
+    const int64_t cap_d = (full < max_bond) ? full : max_bond;
+    D = (l < cap_d) ? l : cap_d;
+    bool found = false;
+    for (int64_t i = 1; i < D; ++i) {
+      if (s[i] <= thr) {
+        D = i;
+        found = true;
+        break;
+      }
+    }
+    // the rule would look past the sketch: only the full spectrum can decide
+    if (!found && D == l && l < cap_d) fallback = 1;
+  }
+  if (D > max_bond) D = max_bond;
+  if (D < 1) D = 1;
+  double e = tail;
+  for (int64_t i = l - 1; i >= D; --i) e += s[i] * s[i];
+  *D_out = D;
+  *err_out = e;
+  *fallback_out = fallback;
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t b = ceil_div(n, 256);
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+constexpr int QRNB = 16;
+
+// Workspace layout of the sketch path (the fallback reuses the whole buffer).
+struct SketchWork {
+  double2 *Om, *Ycm, *Vall, *Tall, *tau, *Z, *ks, *Qcm, *Zcm, *B, *Ub, *VHb, *U, *red, *frob2;
+  double* S;
+  int64_t* Ddev;
+  double* edev;
+  int* fdev;
+  void* svdw;
+  int64_t svd_bytes;
+  int64_t bytes;
+};
+
+// base == nullptr only sizes the layout (w.bytes).
+SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
+  const int64_t mx = m > n ? m : n;
+  SketchWork w;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    void* q = base ? (void*)((char*)base + off) : nullptr;
+    off += al256(bytes);
+    return q;
+  };
+  w.Om = (double2*)take(l * n * 16);
+  w.Ycm = (double2*)take(l * m * 16);
+  w.Vall = (double2*)take(l * mx * 16);
+  w.Tall = (double2*)take(ceil_div(l, QRNB) * QRNB * QRNB * 16);
+  w.tau = (double2*)take(l * 16);
+  w.Z = (double2*)take(2 * QRNB * (mx > l ? mx : l) * 16);
+  w.ks = (double2*)take(8 * QRNB * l * 16);
+  w.Qcm = (double2*)take(l * m * 16);
+  w.Zcm = (double2*)take(l * n * 16);
+  w.B = (double2*)take(l * n * 16);
+  w.Ub = (double2*)take(l * l * 16);
+  w.VHb = (double2*)take(l * n * 16);
+  w.U = (double2*)take(m * l * 16);
+  w.red = (double2*)take(reduce_blocks(m * n) * 16);
+  w.frob2 = (double2*)take(16);
+  w.S = (double*)take(l * 8);
+  w.Ddev = (int64_t*)take(8);
+  w.edev = (double*)take(8);
+  w.fdev = (int*)take(8);
+  w.svd_bytes = svd_workspace_bytes(l, n);
+  w.svdw = take(w.svd_bytes);
+  w.bytes = off;
+  return w;
+}
+
+// Orthonormal basis of the l columns (length len) of Xcm, written to Qout
+// (column-major, l columns of length len).  Xcm is overwritten.
+int orthonormalize(double2* Xcm, int64_t len, int64_t l, double2* Qout, const SketchWork& w,
+                   hipStream_t stream) {
+  int st;
+  if ((st = qr_factor(Xcm, len, l, QRNB, w.Vall, w.Tall, w.tau, w.Z, w.ks, 8 * QRNB * l, stream)))
+    return st;
+  hipLaunchKernelGGL(identity_cm, dim3(grid_for(len * l)), dim3(256), 0, stream, len, l, Qout);
+  if ((st = check_launch("identity_cm"))) return st;
+  return qr_apply_q(Qout, len, l, l, QRNB, w.Vall, w.Tall, w.Z, w.ks, 8 * QRNB * l, stream);
+}
+
+constexpr int NF = 32;  // sketch columns = features of the rank predictor
+
+// Features of the rank predictor from the 32 x 32 Gram matrix G = Y^H Y of the
+// sketch Y = A Omega: eigenvalues by cyclic Jacobi sweeps (one workgroup, the
+// block-Jacobi inner solver), sorted; f[0] = log10(min(m, n)) / 4,
+// f[i] = clamp(log10(sigma_i / sigma_0), -8, 0) / 8 for i = 1..31 (the Gram
+// of the sketch resolves ratios down to ~sqrt(eps), hence the clamp).
+__global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __restrict__ G,
+                                                              int64_t r, float* __restrict__ f) {
+  using namespace jac;
+  __shared__ double2 A[PW * GP];
+  __shared__ double2 A2[PW * GP];
+  __shared__ double2 Q[PW * GP];
+  __shared__ double2 rot[PW];
+  __shared__ int active[2];
+  __shared__ double lam[PW];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < PW * PW; e += blockDim.x) {
+    const int i = e / PW, j = e % PW;
+    A[i * GP + j] = G[e];
+    Q[i * GP + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  double2* cur = A;
+  for (int sw = 0; sw < 12; ++sw) {
+    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-15, 0.0, 1);
+    __syncthreads();  // the next call rewrites the round flags
+  }
+  if (tid < PW) lam[tid] = cur[tid * GP + tid].x;
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 1; i < PW; ++i) {  // insertion sort, descending
+      const double x = lam[i];
+      int j = i - 1;
+      while (j >= 0 && lam[j] < x) {
+        lam[j + 1] = lam[j];
+        --j;
+      }
+      lam[j + 1] = x;
+    }
+    f[0] = (float)(log10((double)r) / 4.0);
+    const double l0 = lam[0];
+    for (int i = 1; i < NF; ++i) {
+      double v = (l0 > 0.0 && lam[i] > 0.0) ? 0.5 * log10(lam[i] / l0) : -8.0;
+      v = v < -8.0 ? -8.0 : (v > 0.0 ? 0.0 : v);
+      f[i] = (float)(v / 8.0);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace seemps
+
+extern "C" int64_t seemps_spectral_sketch_workspace(int64_t m, int64_t n) {
+  auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  return al(seemps::NF * n * 16) + al(seemps::NF * m * 16) + al(seemps::NF * seemps::NF * 16);
+}
+
+extern "C" int seemps_spectral_sketch(int64_t m, int64_t n, const void* A, uint64_t seed,
+                                      float* features, void* work, int64_t work_bytes,
+                                      void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "spectral_sketch: empty matrix");
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_spectral_sketch_workspace(m, n),
+                   "spectral_sketch: workspace too small");
+  hipStream_t st_ = (hipStream_t)stream;
+  auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+  char* p = (char*)work;
+  double2* Om = (double2*)p;
+  p += al(NF * n * 16);
+  double2* Yt = (double2*)p;
+  p += al(NF * m * 16);
+  double2* G = (double2*)p;
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  int st;
+  hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(NF * n)), dim3(256), 0, st_, NF * n, seed, Om);
+  if ((st = check_launch("gauss"))) return st;
+  // Y^T = Omega^T A^T (32 x m); G = conj(Y^T) (Y^T)^T = Y^H Y
+  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, NF, m, n, one, Om, n, 0, A, n, 0, zero, Yt, m, 0, 1,
+                  st_)))
+    return st;
+  if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_T, NF, NF, m, one, Yt, m, 0, Yt, m, 0, zero, G, NF, 0, 1,
+                  st_)))
+    return st;
+  hipLaunchKernelGGL(sketch_features_kernel, dim3(1), dim3(256), 0, st_, G, m < n ? m : n,
+                     features);
+  return check_launch("sketch_features");
+}
+
+extern "C" int64_t seemps_svd_split_sketch_workspace(int64_t m, int64_t n, int64_t l) {
+  const int64_t full = seemps::svd_split_bytes(m, n);
+  const int64_t r = m < n ? m : n;
+  if (l < 1 || l > r) return full;
+  const int64_t sk = seemps::sketch_carve(m, n, l, nullptr).bytes;
+  return sk > full ? sk : full;
+}
+
+extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int mode, int method,
+                                       double tolerance, int64_t max_bond, int64_t l,
+                                       int64_t power_iters, uint64_t seed, void* outA,
+                                       void* outB, double* s_out, int64_t* D_host,
+                                       double* err_host, int32_t* fallback_host, void* work,
+                                       int64_t work_bytes, int32_t* info, void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd_split_sketch: empty matrix");
+  SEEMPS_CHECK_ARG(mode == 0 || mode == 1, "svd_split_sketch: invalid mode");
+  SEEMPS_CHECK_ARG(power_iters >= 0 && power_iters <= 8, "svd_split_sketch: invalid power_iters");
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_sketch_workspace(m, n, l),
+                   "svd_split_sketch: workspace too small");
+  hipStream_t st_ = (hipStream_t)stream;
+  const double2* A = (const double2*)A_;
+  const int64_t r = m < n ? m : n;
+  if (fallback_host) *fallback_host = 0;
+  auto full_split = [&]() {
+    if (fallback_host) *fallback_host = 1;
+    return svd_split_impl(m, n, A, mode, method, tolerance, max_bond, (double2*)outA,
+                          (double2*)outB, s_out, D_host, err_host, work, work_bytes, info, st_);
+  };
+  // nothing to gain from a sketch as wide as the matrix, or no rule to decide with
+  if (l < 1 || l >= r || method == SEEMPS_TRUNC_NONE) return full_split();
+  const SketchWork w = sketch_carve(m, n, l, work);
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  int st;
+  // Y^T = Omega^T A^T, stored column-major (l columns of length m)
+  hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(l * n)), dim3(256), 0, st_, l * n, seed, w.Om);
+  if ((st = check_launch("gauss"))) return st;
+  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.Om, n, 0, A, n, 0, zero, w.Ycm, m, 0, 1,
+                  st_)))
+    return st;
+  for (int64_t it = 0; it < power_iters; ++it) {
+    if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
+    // Z^T = Q^T conj(A): the columns of A^H Q
+    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.Zcm, n,
+                    0, 1, st_)))
+      return st;
+    if ((st = orthonormalize(w.Zcm, n, l, w.B, w, st_))) return st;  // B holds Qz here
+    // Y^T = Qz^T A^T
+    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.B, n, 0, A, n, 0, zero, w.Ycm, m, 0,
+                    1, st_)))
+      return st;
+  }
+  if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
+  // B = Q^H A (l x n)
+  if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_N, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.B, n, 0, 1,
+                  st_)))
+    return st;
+  int off = 0;
+  if ((st = svd_enqueue(l, n, w.B, n, w.Ub, w.S, w.VHb, w.svdw, w.svd_bytes, st_, &off, false)))
+    return st;
+  if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
+  hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
+                     method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
+  if ((st = check_launch("truncate_sketch"))) return st;
+  if ((st = svd_complete_enqueue(l, n, w.Ub, w.VHb, w.svdw, w.Ddev, st_))) return st;
+  int64_t hD = 0;
+  double herr = 0.0;
+  int hfb = 0;
+  SEEMPS_HIP(hipMemcpyAsync(&hD, w.Ddev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(&herr, w.edev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(&hfb, w.fdev, sizeof(int), hipMemcpyDeviceToHost, st_));
+  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;
+  if (hfb) return full_split();
+  const int64_t D = hD;
+  // U = Q U_B (m x l): (Q^T)^T U_B with Q^T stored l x m
+  if ((st = zgemm(SEEMPS_OP_T, SEEMPS_OP_N, m, D, l, one, w.Qcm, m, 0, w.Ub, l, 0, zero, w.U, l, 0,
+                  1, st_)))
+    return st;
+  if (mode == 0) {
+    if ((st = zscale2d_impl(m, D, nullptr, nullptr, w.U, l, (double2*)outA, D, st_))) return st;
+    if ((st = zscale2d_impl(D, n, w.S, nullptr, w.VHb, n, (double2*)outB, n, st_))) return st;
+  } else {
+    if ((st = zscale2d_impl(m, D, nullptr, w.S, w.U, l, (double2*)outA, D, st_))) return st;
+    if ((st = zscale2d_impl(D, n, nullptr, nullptr, w.VHb, n, (double2*)outB, n, st_))) return st;
+  }
+  if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, w.S, l * 8, hipMemcpyDeviceToDevice, st_));
+  *D_host = D;
+  *err_host = herr;
+  return SEEMPS_OK;
+}

@@ -29,20 +29,17 @@
 #include <cstdlib>
 
 #include "common.h"
+#include "jacobi_inner.h"
 
 namespace seemps {
 
-int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
-              double2* tau, double2* Z, double2* ks, int64_t ks_elems, hipStream_t stream);
-int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,
-               const double2* Tall, double2* Z, double2* ks, int64_t ks_elems,
-               hipStream_t stream);
-
 namespace {
 
-constexpr int BW = 16;          // block width (columns)
-constexpr int PW = 2 * BW;      // pair width
-constexpr int GP = PW + 1;      // padded LDS row stride of G and Q (complex)
+using jac::GP;
+using jac::inner_jacobi;
+using jac::PW;
+
+constexpr int BW = PW / 2;      // block width (columns); PW, GP from jacobi_inner.h
 constexpr int RT = 32;          // rows per LDS tile
 constexpr int RTP = RT + 1;     // padded tile row length (complex)
 constexpr int RMAX = 16;        // max row chunks per pair
@@ -331,168 +328,6 @@ __device__ __forceinline__ void pair_blocks(int64_t nb, int64_t k, int64_t p, in
 // Column index (in W / V) of local column t (0 <= t < PW) of the pair.
 __device__ __forceinline__ int64_t pair_col(int64_t bi, int64_t bj, int t) {
   return (t < BW) ? bi * BW + t : bj * BW + (t - BW);
-}
-
-// 1/sqrt(x) and 1/x for positive normal x: the hardware estimates refined by
-// two Newton steps each (quadratic convergence from the estimate's error to a
-// few ulp), so a rotation costs four short FMA chains instead of the longer
-// correctly rounded sqrt / division sequences.
-__device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const double e = fma(-(x * y), y, 1.0);  // 1 - x y^2
-    y = fma(0.5 * y, e, y);
-  }
-  return y;
-}
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const double e = fma(-x, y, 1.0);  // 1 - x y
-    y = fma(y, e, y);
-  }
-  return y;
-}
-
-// Rotation of the index pair (a, b) of the Hermitian matrix G (row stride GP)
-// that annihilates G[a][b]: J = [[cs, sn ph], [-sn conj(ph), cs]], applied as
-// G <- J^H G J.  Identity (sn = 0) when the pair is already orthogonal to
-// tolerance (|G_ab| <= tol sqrt(G_aa G_bb)) or either column is negligible.
-// t = tan(angle) solves t^2 + 2 tau t - 1 = 0 (smaller root); cs = 1/sqrt(1+t^2)
-// and sn = t cs keep cs^2 + sn^2 = 1 to a few ulp for any t.
-__device__ __forceinline__ void jacobi_rotation(const double2* G, int a, int b, double tol,
-                                                double tiny2, double* cs, double* sn,
-                                                double2* ph) {
-  const double gaa = G[a * GP + a].x, gbb = G[b * GP + b].x;
-  const double2 c = G[a * GP + b];
-  const double s2 = cabs2(c);
-  *cs = 1.0;
-  *sn = 0.0;
-  *ph = make_double2(1.0, 0.0);
-  if (s2 > 0.0 && gaa > tiny2 && gbb > tiny2 && s2 > (tol * tol * gaa) * gbb) {
-    const double r = rsqrt_nr(s2);  // 1 / |G_ab|
-    *ph = make_double2(c.x * r, c.y * r);
-    const double tau = (gbb - gaa) * (0.5 * r);
-    double t_;
-    if (fabs(tau) > 1e150) {
-      t_ = 0.5 * rcp_nr(tau);
-    } else {
-      const double q = fma(tau, tau, 1.0);
-      const double den = fabs(tau) + q * rsqrt_nr(q);
-      t_ = (tau >= 0.0 ? 1.0 : -1.0) * rcp_nr(den);
-    }
-    *cs = rsqrt_nr(fma(t_, t_, 1.0));
-    *sn = t_ * (*cs);
-  }
-}
-
-// Index pair t (0 <= t < PW/2) of round k of the circle schedule over PW indices.
-__device__ __forceinline__ void inner_pair(int k, int t, int* a, int* b) {
-  auto idx = [&](int x) { return 1 + (x + k) % (PW - 1); };
-  if (t == 0) {
-    *a = 0;
-    *b = idx(0);
-  } else {
-    *a = idx(t);
-    *b = idx(PW - 1 - t);
-  }
-}
-
-// Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS, row
-// stride GP; G2 is a second buffer of the same shape), accumulating Q (LDS,
-// on entry identity).  256 threads.  Each round: the first 16 lanes derive the
-// round's 16 rotations into rot (one FP64 chain on the critical path instead
-// of one per wavefront); then thread (p, q) applies J_p^H G[pair p, pair q]
-// J_q to one 2x2 block, reading G and writing the other buffer, and updates
-// two rows of Q for pair p; a round whose 16 rotations are all the identity
-// skips that phase.  Returns the buffer holding the final G.
-__device__ double2* inner_jacobi(double2* G, double2* G2, double2* Q, double2* rot, int* active,
-                                 double tol, double tiny2, int sweeps) {
-  const int tid = threadIdx.x;
-  const int p = tid >> 4, q = tid & 15;
-  double2* cur = G;
-  double2* nxt = G2;
-  int rnd = 0;
-  for (int sw = 0; sw < sweeps; ++sw) {
-    for (int k = 0; k < PW - 1; ++k) {
-      if (tid < 64) {
-        int any = 0;
-        if (tid < PW / 2) {
-          int a, b;
-          inner_pair(k, tid, &a, &b);
-          double cs, sn;
-          double2 ph;
-          jacobi_rotation(cur, a, b, tol, tiny2, &cs, &sn, &ph);
-          rot[2 * tid] = make_double2(cs, sn);
-          rot[2 * tid + 1] = ph;
-          any = (sn != 0.0);
-        }
-        any = __any(any);
-        if (tid == 0) active[rnd & 1] = any;
-      }
-      __syncthreads();
-      // identity round: G and Q unchanged, buffers not swapped.  The flag is
-      // double-buffered so a wavefront still reading it is never overtaken.
-      if (!active[(rnd++) & 1]) continue;
-      int a, b, c, d;
-      inner_pair(k, p, &a, &b);
-      inner_pair(k, q, &c, &d);
-      const double2 rp = rot[2 * p], php = rot[2 * p + 1];
-      const double2 rq = rot[2 * q], phq = rot[2 * q + 1];
-      const double csp = rp.x, snp = rp.y, csq = rq.x, snq = rq.y;
-      // Y = X J_q on the block rows (a, b), columns (c, d)
-      const double2 xac = cur[a * GP + c], xad = cur[a * GP + d];
-      const double2 xbc = cur[b * GP + c], xbd = cur[b * GP + d];
-      const double2 pqc = make_double2(phq.x, -phq.y);
-      auto right = [&](double2 xc, double2 xd, double2* yc, double2* yd) {
-        const double2 t1 = cmul(pqc, xd), t2 = cmul(phq, xc);
-        *yc = make_double2(csq * xc.x - snq * t1.x, csq * xc.y - snq * t1.y);
-        *yd = make_double2(snq * t2.x + csq * xd.x, snq * t2.y + csq * xd.y);
-      };
-      double2 yac, yad, ybc, ybd;
-      right(xac, xad, &yac, &yad);
-      right(xbc, xbd, &ybc, &ybd);
-      // Z = J_p^H Y
-      const double2 ppc = make_double2(php.x, -php.y);
-      auto left = [&](double2 ya, double2 yb, double2* za, double2* zb) {
-        const double2 t1 = cmul(php, yb), t2 = cmul(ppc, ya);
-        *za = make_double2(csp * ya.x - snp * t1.x, csp * ya.y - snp * t1.y);
-        *zb = make_double2(snp * t2.x + csp * yb.x, snp * t2.y + csp * yb.y);
-      };
-      double2 zac, zbc, zad, zbd;
-      left(yac, ybc, &zac, &zbc);
-      left(yad, ybd, &zad, &zbd);
-      if (p == q && snp != 0.0) {
-        // (c, d) == (a, b): the rotated pair is exactly orthogonal
-        zad = make_double2(0.0, 0.0);
-        zbc = make_double2(0.0, 0.0);
-        zac.y = 0.0;
-        zbd.y = 0.0;
-      }
-      nxt[a * GP + c] = zac;
-      nxt[a * GP + d] = zad;
-      nxt[b * GP + c] = zbc;
-      nxt[b * GP + d] = zbd;
-      // Q <- Q J_p on rows 2q, 2q + 1
-      if (snp != 0.0) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int x = 2 * q + u;
-          const double2 xa = Q[x * GP + a], xb = Q[x * GP + b];
-          const double2 tb = cmul(ppc, xb), ta = cmul(php, xa);
-          Q[x * GP + a] = make_double2(csp * xa.x - snp * tb.x, csp * xa.y - snp * tb.y);
-          Q[x * GP + b] = make_double2(snp * ta.x + csp * xb.x, snp * ta.y + csp * xb.y);
-        }
-      }
-      __syncthreads();
-      double2* tmp = cur;
-      cur = nxt;
-      nxt = tmp;
-    }
-  }
-  return cur;
 }
 
 // Rotate rows [w0, w1) of the pair's PW columns of W (column stride ldw) and
@@ -1117,7 +952,7 @@ unsigned blocks_for(int64_t n) {
 // svd_complete_enqueue must follow, limited to the slots the caller keeps.
 int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
                 double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
-                int* info_dev_off, bool complete = true) {
+                int* info_dev_off, bool complete) {
   DeviceInfo di;
   int st = device_info(&di);
   if (st) return st;
@@ -1282,21 +1117,21 @@ extern "C" int seemps_truncate(int64_t N, const double* s, int method, double to
                                   (hipStream_t)stream);
 }
 
-extern "C" int64_t seemps_svd_split_workspace(int64_t m, int64_t n) {
+namespace seemps {
+
+int64_t svd_split_bytes(int64_t m, int64_t n) {
   const int64_t r = m < n ? m : n;
   auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
-  return al(m * r * 16) + al(r * 8) + al(r * n * 16) + al(64) + seemps::svd_workspace_bytes(m, n);
+  return al(m * r * 16) + al(r * 8) + al(r * n * 16) + al(64) + svd_workspace_bytes(m, n);
 }
 
-extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, int method,
-                                double tolerance, int64_t max_bond, void* outA, void* outB,
-                                double* s_out, int64_t* D_host, double* err_host, void* work,
-                                int64_t work_bytes, int32_t* info, void* stream) {
+int svd_split_impl(int64_t m, int64_t n, const double2* A, int mode, int method, double tolerance,
+                   int64_t max_bond, double2* outA, double2* outB, double* s_out,
+                   int64_t* D_host, double* err_host, void* work, int64_t work_bytes,
+                   int32_t* info, hipStream_t st_) {
   SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd_split: empty matrix");
   SEEMPS_CHECK_ARG(mode == 0 || mode == 1, "svd_split: invalid mode");
-  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_workspace(m, n),
-                   "svd_split: workspace too small");
-  hipStream_t st_ = (hipStream_t)stream;
+  SEEMPS_CHECK_ARG(work_bytes >= svd_split_bytes(m, n), "svd_split: workspace too small");
   const int64_t r = m < n ? m : n;
   auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
   char* p = (char*)work;
@@ -1310,29 +1145,40 @@ extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, i
   double* edev = (double*)(p + 8);
   p += al(64);
   int off = 0;
-  int st = seemps::svd_enqueue(m, n, (const double2*)A, n, U, S, VH, p,
-                               work_bytes - (p - (char*)work), st_, &off, false);
+  int st = svd_enqueue(m, n, A, n, U, S, VH, p, work_bytes - (p - (char*)work), st_, &off, false);
   if (st) return st;
-  if ((st = seemps::truncate_enqueue(r, S, method, tolerance, max_bond, Ddev, edev, st_)))
-    return st;
-  if ((st = seemps::svd_complete_enqueue(m, n, U, VH, p, Ddev, st_))) return st;
+  if ((st = truncate_enqueue(r, S, method, tolerance, max_bond, Ddev, edev, st_))) return st;
+  if ((st = svd_complete_enqueue(m, n, U, VH, p, Ddev, st_))) return st;
   int64_t hD = 0;
   double herr = 0.0;
   SEEMPS_HIP(hipMemcpyAsync(&hD, Ddev, 8, hipMemcpyDeviceToHost, st_));
   SEEMPS_HIP(hipMemcpyAsync(&herr, edev, 8, hipMemcpyDeviceToHost, st_));
-  if ((st = seemps::svd_read_ctrl(p, off, info, st_))) return st;
+  if ((st = svd_read_ctrl(p, off, info, st_))) return st;
   const int64_t D = hD;
   if (mode == 0) {
-    if ((st = seemps::zscale2d_impl(m, D, nullptr, nullptr, U, r, (double2*)outA, D, st_)))
-      return st;
-    if ((st = seemps::zscale2d_impl(D, n, S, nullptr, VH, n, (double2*)outB, n, st_))) return st;
+    if ((st = zscale2d_impl(m, D, nullptr, nullptr, U, r, outA, D, st_))) return st;
+    if ((st = zscale2d_impl(D, n, S, nullptr, VH, n, outB, n, st_))) return st;
   } else {
-    if ((st = seemps::zscale2d_impl(m, D, nullptr, S, U, r, (double2*)outA, D, st_))) return st;
-    if ((st = seemps::zscale2d_impl(D, n, nullptr, nullptr, VH, n, (double2*)outB, n, st_)))
-      return st;
+    if ((st = zscale2d_impl(m, D, nullptr, S, U, r, outA, D, st_))) return st;
+    if ((st = zscale2d_impl(D, n, nullptr, nullptr, VH, n, outB, n, st_))) return st;
   }
   if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, S, r * 8, hipMemcpyDeviceToDevice, st_));
   *D_host = D;
   *err_host = herr;
   return SEEMPS_OK;
+}
+
+}  // namespace seemps
+
+extern "C" int64_t seemps_svd_split_workspace(int64_t m, int64_t n) {
+  return seemps::svd_split_bytes(m, n);
+}
+
+extern "C" int seemps_svd_split(int64_t m, int64_t n, const void* A, int mode, int method,
+                                double tolerance, int64_t max_bond, void* outA, void* outB,
+                                double* s_out, int64_t* D_host, double* err_host, void* work,
+                                int64_t work_bytes, int32_t* info, void* stream) {
+  return seemps::svd_split_impl(m, n, (const double2*)A, mode, method, tolerance, max_bond,
+                                (double2*)outA, (double2*)outB, s_out, D_host, err_host, work,
+                                work_bytes, info, (hipStream_t)stream);
 }

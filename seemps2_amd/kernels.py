@@ -182,6 +182,32 @@ def svd_split(AA: torch.Tensor, m: int, n: int, mode: int, method: int, tol: flo
             None if s is None else s[:Dv])
 
 
+def svd_split_sketch(AA: torch.Tensor, m: int, n: int, mode: int, method: int, tol: float,
+                     max_bond: int, sketch: int, power_iters: int = 1, seed: int = 0,
+                     want_s: bool = False):
+    """Randomized fixed-rank variant of :func:`svd_split` with sketch size `sketch`.
+
+    Returns (A, B, D, err^2, s or None, fallback) where fallback is True when the
+    full SVD decided the split (the sketch missed more weight than the rule
+    may discard, or `sketch` >= min(m, n))."""
+    r = min(m, n)
+    outA = empty(m * r)
+    outB = empty(r * n)
+    s = torch.empty(r, dtype=torch.float64, device=AA.device) if want_s else None
+    D = ctypes.c_int64(0)
+    err = ctypes.c_double(0.0)
+    fb = ctypes.c_int32(0)
+    info = (ctypes.c_int32 * 2)()
+    wp, wn = workspace.get(lib.seemps_svd_split_sketch_workspace(m, n, sketch))
+    check(lib.seemps_svd_split_sketch(m, n, ptr(AA), mode, method, tol, max_bond, sketch,
+                                      power_iters, seed & 0xFFFFFFFFFFFFFFFF, ptr(outA),
+                                      ptr(outB), ptr(s), ctypes.byref(D), ctypes.byref(err),
+                                      ctypes.byref(fb), wp, wn, info, stream()))
+    Dv = D.value
+    return (outA[: m * Dv].view(m, Dv), outB[: Dv * n].view(Dv, n), Dv, err.value,
+            None if s is None else s[:Dv], bool(fb.value))
+
+
 def permute(x: torch.Tensor, perm) -> torch.Tensor:
     dims = (ctypes.c_int64 * x.dim())(*x.shape)
     p = (ctypes.c_int * x.dim())(*perm)

@@ -197,3 +197,64 @@ def test_two_site_matvec_splitk_shapes(K, rng, chiL, chiR):
     Hm = K.two_site_operator(K.as_dev(O1), K.as_dev(O2))
     out = K.two_site_matvec(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d)
     assert rel(K.to_numpy(out), ref) < 1e-13
+
+
+def _decaying(rng, m, n, rate):
+    U0, _ = np.linalg.qr(crandn(rng, m, min(m, n)))
+    V0, _ = np.linalg.qr(crandn(rng, n, min(m, n)))
+    s0 = np.exp(-rate * np.arange(min(m, n)))
+    return (U0 * s0) @ V0.conj().T, s0
+
+
+def _sketch_bound(s, D, l, q):
+    """Randomized-range accuracy of the kept rank-D part (relative to ||A||):
+    ~ sigma_l (sigma_l / sigma_(D-1))^(2q), with a safety factor of 10."""
+    if l >= s.size:
+        return 1e-12
+    return max(1e-9, 10 * s[l] / s[0] * (s[l] / s[D - 1]) ** (2 * q))
+
+
+@pytest.mark.parametrize("m,n,rate,tol,mb", [(128, 128, 0.15, 1e-10, 1000), (256, 192, 0.1, 1e-8, 64),
+                                             (96, 160, 0.3, 1e-12, 1000)])
+def test_svd_split_sketch_matches_full(K, rng, m, n, rate, tol, mb):
+    """Sketched split: same D as the full split when the sketch holds the kept rank,
+    the kept part within the randomized-range bound (1e-9 when the spectrum
+    has decayed past the sketch), fallback when it does not hold the rank, and
+    the bond cap decided from the sketch alone."""
+    A, sv = _decaying(rng, m, n, rate)
+    st = ocore.Strategy(tolerance=tol, max_bond_dimension=mb)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), st)
+    Dref = Ar.shape[-1]
+    l = min(Dref + 16, min(m, n) - 1)
+    for mode in (0, 1):
+        # accuracy of the kept part ~ sigma_l (sigma_l / sigma_D)^(2q): q = 2 reaches 1e-9 here
+        Ad, Bd, D, err2, _, fb = K.svd_split_sketch(K.as_dev(A), m, n, mode, st.method,
+                                                    st.tolerance, st.max_bond_dimension, l,
+                                                    power_iters=2, seed=7)
+        assert not fb
+        assert D == Dref
+        # the unsketched weight is ||A||^2 - sum s^2: an estimate, good to ~1e-3 here
+        assert abs(np.sqrt(err2) - er) <= 1e-3 * max(er, 1e-12) + 1e-14
+        rec = K.to_numpy(Ad) @ K.to_numpy(Bd)
+        assert rel(rec, Ar.reshape(m, -1) @ Br.reshape(D, -1)) < _sketch_bound(sv, D, l, 2)
+        iso = K.to_numpy(Ad).conj().T @ K.to_numpy(Ad) if mode == 0 else \
+            K.to_numpy(Bd) @ K.to_numpy(Bd).conj().T
+        assert np.abs(iso - np.eye(D)).max() < 1e-12
+    # a sketch far below the kept rank must fall back to the full split
+    Ad, Bd, D, err2, _, fb = K.svd_split_sketch(K.as_dev(A), m, n, 0, st.method, st.tolerance,
+                                                st.max_bond_dimension, max(2, Dref // 3),
+                                                power_iters=1, seed=7)
+    assert fb and D == Dref
+    # bond cap below the sketch size: decided without the full SVD
+    cap = max(2, Dref // 2)
+    stc = ocore.Strategy(tolerance=tol, max_bond_dimension=cap)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), stc)
+    # (the randomized range loses accuracy as sigma_l / sigma_cap grows: a wider
+    # sketch and two power iterations keep this case at 1e-8)
+    Ad, Bd, D, err2, _, fb = K.svd_split_sketch(K.as_dev(A), m, n, 0, stc.method, stc.tolerance,
+                                                cap, min(cap + 32, min(m, n) - 1), power_iters=2,
+                                                seed=3)
+    lc = min(cap + 32, min(m, n) - 1)
+    assert not fb and D == cap == Ar.shape[-1]
+    assert rel(K.to_numpy(Ad) @ K.to_numpy(Bd), Ar.reshape(m, -1) @ Br.reshape(cap, -1)) < \
+        _sketch_bound(sv, cap, lc, 2)

@@ -1,0 +1,142 @@
+"""Learned-rank truncation on the device: MLP kernels against the oracle,
+spectral-sketch features, training, and what the trained predictor gains."""
+
+import numpy as np
+import pytest
+
+from oracle import core as ocore
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def L():
+    from seemps2_amd import learned
+
+    return learned
+
+
+@pytest.fixture(scope="module")
+def K():
+    from seemps2_amd import kernels
+
+    return kernels
+
+
+def crandn(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def decaying(rng, m, n, rate):
+    r = min(m, n)
+    U0, _ = np.linalg.qr(crandn(rng, m, r))
+    V0, _ = np.linalg.qr(crandn(rng, n, r))
+    s = np.exp(-rate * np.arange(r))
+    return (U0 * s) @ V0.conj().T, s
+
+
+def test_mlp_kernels_match_oracle(L, rng):
+    F, H, B = L.F_IN, L.HIDDEN, 37
+    p = omlp.init_params(F, H, rng)
+    p += (0.05 * rng.normal(size=p.size)).astype(np.float32)
+    X = rng.normal(size=(B, F)).astype(np.float32)
+    t = rng.uniform(size=B).astype(np.float32)
+    net = L.RankPredictor(params=p)
+    Xd = torch.tensor(X, device="cuda")
+    y = net.predict(Xd).cpu().numpy()
+    _, _, y_ref = omlp.forward(p, X, F, H)
+    assert np.abs(y - y_ref).max() < 1e-5 * max(1.0, np.abs(y_ref).max())
+    loss, g = net.loss_and_grad(Xd, torch.tensor(t, device="cuda"))
+    loss_ref, g_ref = omlp.loss_and_grad(p, X, t, F, H)
+    assert abs(float(loss.item()) - loss_ref) < 1e-5 * max(1.0, loss_ref)
+    g = g.cpu().numpy()
+    assert np.abs(g - g_ref).max() < 2e-5 * max(1.0, np.abs(g_ref).max())
+    # one Adam step
+    from seemps2_amd._lib import check, lib, ptr, stream
+
+    m = torch.zeros_like(net.params)
+    v = torch.zeros_like(net.params)
+    gd = torch.tensor(g_ref.astype(np.float32), device="cuda")
+    check(lib.seemps_adam_step(net.params.numel(), ptr(net.params), ptr(gd), ptr(m), ptr(v), 1e-3,
+                               0.9, 0.999, 1e-8, 1, stream()))
+    p1_ref, _, _ = omlp.adam(p.astype(np.float64), g_ref.astype(np.float32).astype(np.float64),
+                             0, 0, 1e-3, 0.9, 0.999, 1e-8, 1)
+    assert np.abs(net.params.cpu().numpy() - p1_ref).max() < 1e-6
+
+
+def test_spectral_features(L, rng):
+    m = n = 96
+    A, s = decaying(rng, m, n, 0.4)
+    f = L.spectral_features(torch.tensor(A, device="cuda"), m, n, seed=3).cpu().numpy()
+    assert abs(f[0] - np.log10(96) / 4) < 1e-6
+    assert np.all(np.diff(f[1:]) <= 1e-6) and np.all(f[1:] <= 0) and np.all(f[1:] >= -1)
+    # a random projection keeps the leading decay: log10(s1/s0) = -0.4/ln 10
+    assert abs(f[1] * 8 - np.log10(s[1] / s[0])) < 0.5
+    # rank one: the other sketch values sit at the Gram's rounding floor (~1e-8)
+    A1 = np.outer(crandn(rng, m), crandn(rng, n))
+    f1 = L.spectral_features(torch.tensor(A1, device="cuda"), m, n, seed=4).cpu().numpy()
+    assert np.all(f1[1:] < -0.9)
+
+
+def _dataset(L, rng, count, tol, size=128):
+    feats, targets, mats = [], [], []
+    st = ocore.Strategy(tolerance=tol)
+    for k in range(count):
+        rate = float(rng.uniform(0.06, 0.6))
+        A, s = decaying(rng, size, size, rate)
+        D, _ = ocore.truncate(s.copy(), st)
+        feats.append(L.spectral_features(torch.tensor(A, device="cuda"), size, size, seed=k))
+        targets.append(D / size)
+        mats.append((A, D))
+    return torch.stack(feats), torch.tensor(targets, dtype=torch.float32, device="cuda"), mats
+
+
+def test_training_and_ablation(L, K, rng):
+    """Trained on logged spectra, the predictor picks sketches far narrower than
+    the width one must use without it, and the kept ranks equal the full split's."""
+    tol = 1e-8
+    X, t, _ = _dataset(L, rng, 96, tol)
+    net = L.RankPredictor(seed=5)
+    losses = net.train(X, t, epochs=500, lr=3e-3)
+    assert losses[-1] < 0.2 * losses[0]
+    assert losses[-1] < 2e-3
+    Xh, th, held = _dataset(L, rng, 24, tol)
+    pred = net.predict(Xh).cpu().numpy()
+    assert np.abs(pred - th.cpu().numpy()).mean() < 0.05
+    strategy = ocore.Strategy(tolerance=tol)
+    lt = L.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=2, min_size=32)
+    widths_learned, widths_plain = [], []
+    for A, D_ref in held:
+        Ad = torch.tensor(A, device="cuda")
+        a_, b_, D, err2 = lt.split(Ad, A.shape[0], A.shape[1], 0, strategy)
+        assert D == D_ref
+        widths_learned.append(lt.sketch_width(L.spectral_features(Ad, *A.shape, seed=1),
+                                              *A.shape, strategy.max_bond_dimension))
+        widths_plain.append(min(A.shape))  # no predictor: only the full width is safe
+    assert lt.fallbacks <= 2
+    assert np.mean(widths_learned) < 0.7 * np.mean(widths_plain)
+
+
+def test_learned_dmrg_energy_matches_plain(L):
+    """DMRG with the learned split agrees with the plain split to 1e-9 (relative)."""
+    import seemps2_amd as S
+
+    N = 12
+    H = S.XXZHamiltonian(N, 1.0, 0.5).to_mpo()
+    rng = np.random.default_rng(7)
+    guess = S.random_mps([2] * N, D=8, complex=True, rng=rng)
+    st = S.DEFAULT_STRATEGY.replace(max_bond_dimension=48, tolerance=1e-10)
+    log = L.SpectrumLog(min_size=8)
+    with L.recording(log):
+        plain = S.dmrg(H, guess, strategy=st, maxiter=6)
+    X, t = log.tensors()
+    net = L.RankPredictor(seed=2)
+    net.train(X, t, epochs=300, lr=3e-3)
+    lt = L.LearnedTruncation(net, min_size=8, oversample=8)
+    with L.learned_truncation(lt):
+        learned = S.dmrg(H, guess, strategy=st, maxiter=6)
+    assert abs(learned.energy - plain.energy) <= 1e-9 * abs(plain.energy)
+    assert lt.calls > 0
