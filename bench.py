@@ -20,6 +20,7 @@ on a bounded sample of the same bond updates.
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import sys
@@ -48,6 +49,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-rank (randomized SVD) form of the timed sweeps")
+    ap.add_argument("--sketch-power-iters", type=int, default=1,
+                    help="power iterations of the learned form's randomized SVD")
     ap.add_argument("--kernel-timing", default=TIMED_KERNELS,
                     help="kernels timed with HIP events inside the timed region ('' = none)")
     ap.add_argument("--rocprof-summary", metavar="DIR",
@@ -185,6 +190,38 @@ def timed_sweeps(QF, strategy, direction, steps, chi):
 TIMED_KERNELS = "svd_jacobi,zgemm_kernel"
 
 
+def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_plain,
+                 power_iters=1):
+    """The same timed sweeps from the same state with the learned-rank split:
+    predictor trained on the warm-up's spectra (device kernels), sketched
+    randomized SVDs with full-SVD fallback.  Returns its rate, the speedup over
+    the plain FP64 path and the energy difference."""
+    from seemps2_amd import learned
+
+    X, t = spectra.tensors()
+    net = learned.RankPredictor(seed=11)
+    t0 = time.perf_counter()
+    losses = net.train(X, t, epochs=400, lr=3e-3)
+    t_train = time.perf_counter() - t0
+    lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=power_iters,
+                                   min_size=64)
+    with learned.learned_truncation(lt):
+        secs, t_chi, n_chi, n_all, matvecs, _, E = timed_sweeps(QF, strategy, direction, steps, chi)
+    value = n_chi / t_chi if t_chi > 0 else 0.0
+    rel = abs(E - E_plain) / max(abs(E_plain), 1e-300)
+    log(f"learned form: {n_chi} chi={chi} updates in {t_chi:.3f} s ({value:.2f}/s, "
+        f"x{value / max(value_plain, 1e-30):.2f}), energy rel. diff {rel:.2e}, "
+        f"{lt.fallbacks} fallbacks of {lt.calls} splits, training {t_train:.2f} s "
+        f"(loss {losses[0]:.2e} -> {losses[-1]:.2e} on {X.shape[0]} spectra)")
+    return dict(value=round(value, 3), unit="bond_updates/s",
+                speedup_vs_plain=round(value / max(value_plain, 1e-30), 3),
+                energy_rel_diff=float(f"{rel:.3e}"), fallbacks=lt.fallbacks, splits=lt.calls,
+                mean_sketch_width=round(lt.sketch_columns / max(lt.calls - lt.full_splits, 1), 1),
+                power_iters=power_iters,
+                training_spectra=int(X.shape[0]), training_seconds=round(t_train, 3),
+                ms_per_step=round(secs / max(steps, 1) * 1e3, 3))
+
+
 def roofline(records):
     """Roofline of the dominant kernel from the in-region HIP-event records."""
     if not records:
@@ -296,9 +333,16 @@ def main():
     t0 = time.perf_counter()
     direction = grow(QF, strategy, chi)
     log(f"grown to chi={max(QF.state.bond_dimensions())} in {time.perf_counter() - t0:.1f} s")
+    from seemps2_amd import learned
+
+    spectra = learned.SpectrumLog(min_size=64)
     if args.warmup > 0:
-        tw, *_rest, direction, _ = timed_sweeps(QF, strategy, direction, args.warmup, chi)
-        log(f"warmup {args.warmup} sweep(s): {tw:.2f} s")
+        # the warm-up's full SVDs also provide the rank predictor's training spectra
+        with learned.recording(spectra):
+            tw, *_rest, direction, _ = timed_sweeps(QF, strategy, direction, args.warmup, chi)
+        log(f"warmup {args.warmup} sweep(s): {tw:.2f} s, {len(spectra.targets)} spectra logged")
+    snapshot = copy.deepcopy(QF) if not args.no_learned else None
+    direction0 = direction
     from seemps2_amd import _lib
 
     _lib.timing_enable(args.kernel_timing or None)
@@ -317,6 +361,10 @@ def main():
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
+    chosen = None
+    if snapshot is not None and len(spectra.targets) >= 16:
+        chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value, E,
+                              args.sketch_power_iters)
     cpu = None
     if not args.no_cpu_baseline:
         start = advance_to_cap(QF, strategy, direction, chi)
@@ -347,6 +395,7 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_form": chosen,
     }
     print(json.dumps(line), flush=True)
 
