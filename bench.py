@@ -53,6 +53,8 @@ def parse():
                     help="skip the learned-rank (randomized SVD) form of the timed sweeps")
     ap.add_argument("--sketch-power-iters", type=int, default=1,
                     help="power iterations of the learned form's randomized SVD")
+    ap.add_argument("--eig-precision", default="mixed", choices=["fp64", "mixed"],
+                    help="eigensolver contractions of the learned form")
     ap.add_argument("--kernel-timing", default=TIMED_KERNELS,
                     help="kernels timed with HIP events inside the timed region ('' = none)")
     ap.add_argument("--rocprof-summary", metavar="DIR",
@@ -149,7 +151,7 @@ def advance_to_cap(QF, strategy, direction, chi):
     return None
 
 
-def timed_sweeps(QF, strategy, direction, steps, chi):
+def timed_sweeps(QF, strategy, direction, steps, chi, precision="fp64"):
     """Run `steps` full sweeps.  Returns (sweep seconds, seconds spent in the
     chi x chi bond updates, their number, all updates, matvecs, direction, energy).
     Each update ends with a device synchronisation (the eigensolver synchronises
@@ -172,7 +174,7 @@ def timed_sweeps(QF, strategy, direction, steps, chi):
             chiL = QF.state[i].shape[0]
             chiR = QF.state[i + 1].shape[2]
             tu = time.perf_counter()
-            E, AB = D._diagonalize_two_site(QF, i, 1e-10)
+            E, AB = D._diagonalize_two_site(QF, i, 1e-10, precision=precision)
             if direction > 0:
                 QF.update_2site_right(AB, i, strategy)
             else:
@@ -191,11 +193,12 @@ TIMED_KERNELS = "svd_jacobi,zgemm_kernel"
 
 
 def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_plain,
-                 power_iters=1):
+                 power_iters=1, precision="mixed"):
     """The same timed sweeps from the same state with the learned-rank split:
     predictor trained on the warm-up's spectra (device kernels), sketched
-    randomized SVDs with full-SVD fallback.  Returns its rate, the speedup over
-    the plain FP64 path and the energy difference."""
+    randomized SVDs with full-SVD fallback, and the eigensolver's contractions in
+    `precision`.  Returns its rate, the speedup over the plain FP64 path and the
+    energy difference."""
     from seemps2_amd import learned
 
     X, t = spectra.tensors()
@@ -205,19 +208,26 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     t_train = time.perf_counter() - t0
     lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=power_iters,
                                    min_size=64)
+    import importlib
+
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
     with learned.learned_truncation(lt):
-        secs, t_chi, n_chi, n_all, matvecs, _, E = timed_sweeps(QF, strategy, direction, steps, chi)
+        secs, t_chi, n_chi, n_all, matvecs, _, E = timed_sweeps(QF, strategy, direction, steps, chi,
+                                                                precision)
+    mv32 = D.STATS.matvecs_fp32
     value = n_chi / t_chi if t_chi > 0 else 0.0
     rel = abs(E - E_plain) / max(abs(E_plain), 1e-300)
     log(f"learned form: {n_chi} chi={chi} updates in {t_chi:.3f} s ({value:.2f}/s, "
         f"x{value / max(value_plain, 1e-30):.2f}), energy rel. diff {rel:.2e}, "
-        f"{lt.fallbacks} fallbacks of {lt.calls} splits, training {t_train:.2f} s "
+        f"{lt.fallbacks} fallbacks of {lt.calls} splits, {mv32}/{matvecs} FP32 matvecs, "
+        f"training {t_train:.2f} s "
         f"(loss {losses[0]:.2e} -> {losses[-1]:.2e} on {X.shape[0]} spectra)")
     return dict(value=round(value, 3), unit="bond_updates/s",
                 speedup_vs_plain=round(value / max(value_plain, 1e-30), 3),
                 energy_rel_diff=float(f"{rel:.3e}"), fallbacks=lt.fallbacks, splits=lt.calls,
                 mean_sketch_width=round(lt.sketch_columns / max(lt.calls - lt.full_splits, 1), 1),
-                power_iters=power_iters,
+                power_iters=power_iters, eigensolver_precision=precision,
+                matvecs=matvecs, matvecs_fp32=mv32,
                 training_spectra=int(X.shape[0]), training_seconds=round(t_train, 3),
                 ms_per_step=round(secs / max(steps, 1) * 1e3, 3))
 
@@ -364,7 +374,7 @@ def main():
     chosen = None
     if snapshot is not None and len(spectra.targets) >= 16:
         chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value, E,
-                              args.sketch_power_iters)
+                              args.sketch_power_iters, args.eig_precision)
     cpu = None
     if not args.no_cpu_baseline:
         start = advance_to_cap(QF, strategy, direction, chi)

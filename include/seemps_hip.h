@@ -42,6 +42,15 @@ extern "C" {
 #define SEEMPS_OP_H 2
 #define SEEMPS_OP_C 3
 
+/* Arithmetic of the contractions (no counterpart in the reference, which is
+ * FP64 throughout).  FP32: operands rounded to FP32 on load, FP32 MFMA products
+ * and accumulation, complex128 in memory. */
+#define SEEMPS_PREC_FP64 0
+#define SEEMPS_PREC_FP32 1
+/* Eigensolver only: FP32 contractions until the Ritz residual reaches
+ * 1e-5 |theta|, FP64 from there to the requested tolerance. */
+#define SEEMPS_PREC_MIXED 2
+
 /* Truncation method codes: Truncation.* of src/seemps/cython/truncation.pxi:1-8. */
 #define SEEMPS_TRUNC_NONE 0
 #define SEEMPS_TRUNC_REL_SV 1
@@ -71,6 +80,17 @@ int seemps_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
                  double beta_re, double beta_im,
                  void* C, int64_t ldc, int64_t strideC,
                  int64_t batch, void* stream);
+
+/* seemps_zgemm with the product in `precision` (SEEMPS_PREC_FP64 or
+ * SEEMPS_PREC_FP32).  No counterpart in the reference: the FP32 form is the
+ * contraction inside the mixed-precision eigensolver. */
+int seemps_zgemm_prec(int precision, int opA, int opB, int64_t M, int64_t N, int64_t K,
+                      double alpha_re, double alpha_im,
+                      const void* A, int64_t lda, int64_t strideA,
+                      const void* B, int64_t ldb, int64_t strideB,
+                      double beta_re, double beta_im,
+                      void* C, int64_t ldc, int64_t strideC,
+                      int64_t batch, void* stream);
 
 /* Local-operator contraction, complex128:
  *   out[p, j1, q, j2] = sum_i in[p, i, q] * Mop[i, j1, j2]   (+ out if accumulate)
@@ -172,8 +192,10 @@ int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, const void* 
  * (a = b = chiL, g = f = chiR), Lanczos with full re-orthogonalisation (classical
  * Gram-Schmidt with the DGKS correction), Krylov dimension ncv (2..23), explicit
  * restarts from the Ritz vector, started from v0; stops when the Ritz residual
- * |beta_m y_m| <= tol * max(eps^(2/3), |theta|) (ARPACK's test; tol = 0 means
- * machine epsilon).  The Krylov recurrence, the tridiagonal eigensolve (FP64
+ * |beta_k y_k| <= tol * max(eps^(2/3), |theta|) (ARPACK's test; tol = 0 means
+ * machine epsilon), tested at the end of each cycle (k = ncv) and, for
+ * operators of dimension >= 8192, also after steps k = 8, 12, 16, ... so a warm
+ * start stops before ncv products.  The Krylov recurrence, the tridiagonal eigensolve (FP64
  * bisection + inverse iteration) and the Ritz vector stay on the device; the
  * stream is synchronised once per restart.  evec receives the unit-norm
  * eigenvector; stats (host, may be NULL) = {eigenvalue, residual, matvecs,
@@ -186,6 +208,20 @@ int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2, 
                                const void* v0, void* evec, int64_t ncv, double tol,
                                int64_t maxiter, double* stats, void* work,
                                int64_t work_bytes, void* stream);
+/* seemps_two_site_eigensolve with the contractions of the operator in
+ * `precision`.  SEEMPS_PREC_MIXED: when the start vector's residual (from the
+ * first, FP64 product) exceeds 1e-4 |alpha_0|, restarts run with FP32
+ * contractions until the Ritz residual reaches 1e-5 max(eps^(2/3), |theta|) (or
+ * stops falling); FP64 restarts then continue to the requested tol, so the
+ * result meets the same test as the FP64 solver.  SEEMPS_PREC_FP32 stays in FP32 (tol below ~1e-6 cannot be met).
+ * Orthogonalisation, tridiagonal solve and Ritz vectors are FP64 in all modes.
+ * stats[4] (when stats has 5 slots, i.e. stats5 != 0) receives the number of
+ * FP32 matvecs.  No counterpart in the reference. */
+int seemps_two_site_eigensolve_prec(int precision, int64_t chiL, int64_t c, int64_t d1,
+                                    int64_t d2, int64_t w2, int64_t chiR, const void* L,
+                                    const void* Hm, const void* R, const void* v0, void* evec,
+                                    int64_t ncv, double tol, int64_t maxiter, double* stats5,
+                                    void* work, int64_t work_bytes, void* stream);
 
 /* Fixed-rank randomized variant of seemps_svd_split (no counterpart in the
  * reference: the learned-rank truncation).  Sketch Y = A Omega with l columns

@@ -39,6 +39,11 @@ _SIGNATURES = {
         c_int,
         [c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _I, _I, _I, _P],
     ),
+    "seemps_zgemm_prec": (
+        c_int,
+        [c_int, c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _I, _I, _I,
+         _P],
+    ),
     "seemps_zlocal": (c_int, [_I, _I, _I, _I, _I, _P, _P, _P, c_int, _P]),
     "seemps_zaxpby": (c_int, [_I, _D, _D, _P, c_int, _D, _D, _P, _P]),
     "seemps_zscale2d": (c_int, [_I, _I, _P, _P, _P, _I, _P, _I, _P]),
@@ -63,6 +68,11 @@ _SIGNATURES = {
     "seemps_two_site_eigensolve": (
         c_int,
         [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _D, _I, POINTER(c_double), _P, _I, _P],
+    ),
+    "seemps_two_site_eigensolve_prec": (
+        c_int,
+        [c_int, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _D, _I, POINTER(c_double), _P,
+         _I, _P],
     ),
     "seemps_svd_split_sketch_workspace": (_I, [_I, _I, _I]),
     "seemps_svd_split_sketch": (

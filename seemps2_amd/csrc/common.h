@@ -88,14 +88,17 @@ struct DeviceInfo {
 int device_info(DeviceInfo* info);
 
 // Internal enqueue functions shared between the translation units.
+// prec: SEEMPS_PREC_FP64 (FP64 MFMA) or SEEMPS_PREC_FP32 (operands rounded to FP32,
+// FP32 MFMA and accumulation; A, B and C stay complex128 in memory).
 int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
           int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB, double2 beta, void* C,
-          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream);
+          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream,
+          int prec = SEEMPS_PREC_FP64);
 // Split-K GEMM over caller scratch (ws holds splitk_workspace(M, N, K) elements);
 // falls back to zgemm when splitting does not pay or ws is too small.
 int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
              int64_t lda, const void* B, int64_t ldb, double2 beta, void* C, int64_t ldc,
-             double2* ws, int64_t ws_elems, hipStream_t stream);
+             double2* ws, int64_t ws_elems, hipStream_t stream, int prec = SEEMPS_PREC_FP64);
 int64_t splitk_workspace(int64_t M, int64_t N, int64_t K);
 int zaxpby_impl(int64_t n, double2 a, const double2* x, int conj_x, double2 b, double2* y,
                 hipStream_t stream);

@@ -536,18 +536,19 @@ unsigned grid_for(int64_t n) {
 int two_site_matvec_impl(int64_t a, int64_t c, int64_t b, int64_t d1, int64_t d2, int64_t w2,
                          int64_t f, int64_t g, const double2* L, const double2* Hm,
                          const double2* R, const double2* v, double2* out, double2* x1,
-                         double2* x2, double2* ks, int64_t ks_elems, hipStream_t stream) {
+                         double2* x2, double2* ks, int64_t ks_elems, hipStream_t stream,
+                         int prec) {
   int st;
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   // x1[(a,c), (k,l,f)] = L[(a,c), b] v[b, (k,l,f)]
   if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, a * c, d1 * d2 * f, b, one, L, b, v, d1 * d2 * f,
-                     zero, x1, d1 * d2 * f, ks, ks_elems, stream)))
+                     zero, x1, d1 * d2 * f, ks, ks_elems, stream, prec)))
     return st;
   // x2[a, (ko,lo,d), f] = sum_{(c,k,l)} x1[a, (c,k,l), f] Hm[(c,k,l), (ko,lo,d)]
   if ((st = zlocal_impl(a, c * d1 * d2, f, d1 * d2 * w2, 1, x1, Hm, x2, 0, stream))) return st;
   // out[(a,ko,lo), g] = x2[(a,ko,lo), (d,f)] R[g, (d,f)]^T
   return zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, a * d1 * d2, g, w2 * f, one, x2, w2 * f, R, w2 * f,
-                  zero, out, g, ks, ks_elems, stream);
+                  zero, out, g, ks, ks_elems, stream, prec);
 }
 
 }  // namespace seemps
@@ -610,7 +611,7 @@ extern "C" int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d
   const int64_t ks_avail = (work_bytes - ((char*)ks - (char*)work)) / 16;
   return two_site_matvec_impl(a, c, b, d1, d2, w2, f, g, (const double2*)L, (const double2*)Hm,
                               (const double2*)R, (const double2*)v, (double2*)out, x1, x2, ks,
-                              ks_avail, (hipStream_t)stream);
+                              ks_avail, (hipStream_t)stream, SEEMPS_PREC_FP64);
 }
 
 extern "C" int64_t seemps_eigensolve_workspace(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
@@ -620,12 +621,15 @@ extern "C" int64_t seemps_eigensolve_workspace(int64_t chiL, int64_t c, int64_t 
   return eig_bytes_full(chiL, c, d1, d2, w2, chiR, m);
 }
 
-extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
-                                          int64_t w2, int64_t chiR, const void* L, const void* Hm,
-                                          const void* R, const void* v0, void* evec, int64_t ncv,
-                                          double tol, int64_t maxiter, double* stats, void* work,
-                                          int64_t work_bytes, void* stream) {
-  hipStream_t st_ = (hipStream_t)stream;
+namespace {
+
+int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, int64_t w2,
+                    int64_t chiR, const void* L, const void* Hm, const void* R, const void* v0,
+                    void* evec, int64_t ncv, double tol, int64_t maxiter, double* stats,
+                    int nstats, void* work, int64_t work_bytes, hipStream_t st_) {
+  SEEMPS_CHECK_ARG(prec == SEEMPS_PREC_FP64 || prec == SEEMPS_PREC_FP32 ||
+                       prec == SEEMPS_PREC_MIXED,
+                   "eigensolve: invalid precision");
   SEEMPS_CHECK_ARG(chiL >= 1 && chiR >= 1 && c >= 1 && w2 >= 1 && d1 >= 1 && d2 >= 1,
                    "eigensolve: bad dimensions");
   SEEMPS_CHECK_ARG(ncv >= 2 && ncv < LZ_KREG, "eigensolve: ncv must be in [2, %d]",
@@ -651,19 +655,36 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
   hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, w.V);
   if ((st = check_launch("lz_unit"))) return st;
   double theta = 0.0, resid = 0.0;
-  int64_t matvecs = 0, cycle = 0;
+  int64_t matvecs = 0, matvecs32 = 0, cycle = 0;
   bool converged = false;
+  // FP32 phase: the operator is applied with FP32 products, whose rounding
+  // (~1e-7 relative) floors the residual near 1e-7 |theta|.  MIXED enters it
+  // only when the start vector's residual (the first step's beta, from an FP64
+  // product) exceeds 1e-4 |alpha|, and leaves it at 1e-5 |theta| or when a
+  // restart gains less than a factor 4.  A good start vector (a converged
+  // DMRG sweep) thus stays in FP64, where one short cycle suffices.
+  int mv_prec = prec == SEEMPS_PREC_FP32 ? SEEMPS_PREC_FP32 : SEEMPS_PREC_FP64;
+  const double fp32_floor = 1e-5, fp32_start = 1e-4;
+  double last_resid = HUGE_VAL;
+  // In FP64 cycles the Ritz residual is also tested after steps 8, 12, 16, ...
+  // (one single-block tridiagonal solve and a synchronisation each), so that a
+  // warm start stops well before ncv products; only for operators large enough
+  // that a product outweighs the check.
+  const bool early = n >= 8192;
+  const int64_t first_check = 8, check_every = 4;
   for (cycle = 0; cycle < maxiter; ++cycle) {
     hipLaunchKernelGGL(lz_reset, dim3(1), dim3(64), 0, st_, w.sc);
+    int64_t kd_eff = m;
     for (int64_t j = 0; j < m; ++j) {
       double2* vj = w.V + j * n;
       double2* wv = w.V + (j + 1) * n;
       if ((st = two_site_matvec_impl(chiL, c, chiL, d1, d2, w2, chiR, chiR, L_, H_, R_, vj, wv,
                                      w.x1, w.x2, w.ks,
                                      matvec_ks_elems(chiL, c, chiL, d1, d2, w2, chiR, chiR),
-                                     st_)))
+                                     st_, mv_prec)))
         return st;
       ++matvecs;
+      if (mv_prec == SEEMPS_PREC_FP32) ++matvecs32;
       // Gram-Schmidt against V[0..j] with the DGKS correction, then the norm:
       // four launches (dots | subtract + dots | conditional subtract | normalise).
       const int kk = (int)(j + 1);
@@ -691,12 +712,37 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
                            wv);
       }
       if ((st = check_launch("lanczos step"))) return st;
+      if (prec == SEEMPS_PREC_MIXED && cycle == 0 && j == 0) {
+        double ab[2];
+        SEEMPS_HIP(hipMemcpyAsync(&ab[0], &w.sc->alpha[0], 8, hipMemcpyDeviceToHost, st_));
+        SEEMPS_HIP(hipMemcpyAsync(&ab[1], &w.sc->beta[0], 8, hipMemcpyDeviceToHost, st_));
+        SEEMPS_HIP(hipStreamSynchronize(st_));
+        if (ab[1] > fp32_start * fmax(eps23, fabs(ab[0]))) mv_prec = SEEMPS_PREC_FP32;
+      }
+      const int64_t kd = j + 1;
+      if (early && mv_prec == SEEMPS_PREC_FP64 && kd < m && kd >= first_check &&
+          (kd - first_check) % check_every == 0) {
+        hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)kd);
+        if ((st = check_launch("lz_tridiag"))) return st;
+        double h[2];
+        SEEMPS_HIP(hipMemcpyAsync(&h[0], &w.sc->theta, 8, hipMemcpyDeviceToHost, st_));
+        SEEMPS_HIP(hipMemcpyAsync(&h[1], &w.sc->resid, 8, hipMemcpyDeviceToHost, st_));
+        SEEMPS_HIP(hipStreamSynchronize(st_));
+        if (h[1] <= tol * fmax(eps23, fabs(h[0]))) {
+          kd_eff = kd;
+          break;
+        }
+      }
     }
-    hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)m);
-    if ((st = check_launch("lz_tridiag"))) return st;
-    // Ritz vector into V[m], normalise, restart from it.
+    if (kd_eff == m) {
+      hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)m);
+      if ((st = check_launch("lz_tridiag"))) return st;
+    }
+    // Ritz vector into V[m] (free: at most V[kd_eff] holds a Lanczos vector),
+    // normalise, restart from it.
     double2* x = w.V + m * n;
-    hipLaunchKernelGGL(combine_kernel, dim3(gb), dim3(256), 0, st_, n, (int)m, w.V, n, w.sc->y, x);
+    hipLaunchKernelGGL(combine_kernel, dim3(gb), dim3(256), 0, st_, n, (int)kd_eff, w.V, n,
+                       w.sc->y, x);
     if ((st = check_launch("combine"))) return st;
     if ((st = zmdotc_impl(n, 1, x, n, x, &w.sc->nrm, w.part, st_))) return st;
     hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, x);
@@ -707,7 +753,19 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
     SEEMPS_HIP(hipStreamSynchronize(st_));
     theta = h[0];
     resid = h[1];
-    if (resid <= tol * fmax(eps23, fabs(theta))) {
+    const double scale = fmax(eps23, fabs(theta));
+    if (mv_prec == SEEMPS_PREC_FP32 && prec == SEEMPS_PREC_MIXED) {
+      // An FP32 residual meets the test only for tolerances FP32 can resolve.
+      if (tol >= fp32_floor && resid <= tol * scale) {
+        converged = true;
+        ++cycle;
+        break;
+      }
+      if (resid <= fp32_floor * scale || resid > 0.25 * last_resid) mv_prec = SEEMPS_PREC_FP64;
+      last_resid = resid;
+      continue;
+    }
+    if (resid <= tol * scale) {
       converged = true;
       ++cycle;
       break;
@@ -719,6 +777,7 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
     stats[1] = resid;
     stats[2] = (double)matvecs;
     stats[3] = (double)cycle;
+    if (nstats > 4) stats[4] = (double)matvecs32;
   }
   if (!converged) {
     set_error("eigensolve: no convergence after %lld restarts (residual %g)", (long long)cycle,
@@ -726,4 +785,25 @@ extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, i
     return SEEMPS_ERR_NOCONVERGE;
   }
   return SEEMPS_OK;
+}
+
+}  // namespace
+
+extern "C" int seemps_two_site_eigensolve(int64_t chiL, int64_t c, int64_t d1, int64_t d2,
+                                          int64_t w2, int64_t chiR, const void* L, const void* Hm,
+                                          const void* R, const void* v0, void* evec, int64_t ncv,
+                                          double tol, int64_t maxiter, double* stats, void* work,
+                                          int64_t work_bytes, void* stream) {
+  return eigensolve_impl(SEEMPS_PREC_FP64, chiL, c, d1, d2, w2, chiR, L, Hm, R, v0, evec, ncv,
+                         tol, maxiter, stats, 4, work, work_bytes, (hipStream_t)stream);
+}
+
+extern "C" int seemps_two_site_eigensolve_prec(int precision, int64_t chiL, int64_t c,
+                                               int64_t d1, int64_t d2, int64_t w2, int64_t chiR,
+                                               const void* L, const void* Hm, const void* R,
+                                               const void* v0, void* evec, int64_t ncv,
+                                               double tol, int64_t maxiter, double* stats5,
+                                               void* work, int64_t work_bytes, void* stream) {
+  return eigensolve_impl(precision, chiL, c, d1, d2, w2, chiR, L, Hm, R, v0, evec, ncv, tol,
+                         maxiter, stats5, 5, work, work_bytes, (hipStream_t)stream);
 }

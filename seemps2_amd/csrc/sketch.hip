@@ -303,32 +303,39 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
   const SketchWork w = sketch_carve(m, n, l, work);
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   int st;
-  // Y^T = Omega^T A^T, stored column-major (l columns of length m)
-  hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(l * n)), dim3(256), 0, st_, l * n, seed, w.Om);
-  if ((st = check_launch("gauss"))) return st;
-  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.Om, n, 0, A, n, 0, zero, w.Ycm, m, 0, 1,
-                  st_)))
-    return st;
-  for (int64_t it = 0; it < power_iters; ++it) {
-    if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
-    // Z^T = Q^T conj(A): the columns of A^H Q
-    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.Zcm, n,
-                    0, 1, st_)))
+  {
+    TimedLaunch phase_range("phase_sketch_range", st_, 0.0);
+    // Y^T = Omega^T A^T, stored column-major (l columns of length m)
+    hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(l * n)), dim3(256), 0, st_, l * n, seed, w.Om);
+    if ((st = check_launch("gauss"))) return st;
+    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.Om, n, 0, A, n, 0, zero, w.Ycm, m, 0, 1,
+                    st_)))
       return st;
-    if ((st = orthonormalize(w.Zcm, n, l, w.B, w, st_))) return st;  // B holds Qz here
-    // Y^T = Qz^T A^T
-    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.B, n, 0, A, n, 0, zero, w.Ycm, m, 0,
-                    1, st_)))
+    for (int64_t it = 0; it < power_iters; ++it) {
+      if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
+      // Z^T = Q^T conj(A): the columns of A^H Q
+      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.Zcm, n,
+                      0, 1, st_)))
+        return st;
+      if ((st = orthonormalize(w.Zcm, n, l, w.B, w, st_))) return st;  // B holds Qz here
+      // Y^T = Qz^T A^T
+      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.B, n, 0, A, n, 0, zero, w.Ycm, m, 0,
+                      1, st_)))
+        return st;
+    }
+    if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
+    // B = Q^H A (l x n)
+    if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_N, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.B, n, 0, 1,
+                    st_)))
       return st;
   }
-  if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
-  // B = Q^H A (l x n)
-  if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_N, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.B, n, 0, 1,
-                  st_)))
-    return st;
   int off = 0;
-  if ((st = svd_enqueue(l, n, w.B, n, w.Ub, w.S, w.VHb, w.svdw, w.svd_bytes, st_, &off, false)))
-    return st;
+  {
+    TimedLaunch phase_svd("phase_sketch_svd", st_, 0.0);
+    if ((st = svd_enqueue(l, n, w.B, n, w.Ub, w.S, w.VHb, w.svdw, w.svd_bytes, st_, &off,
+                          false)))
+      return st;
+  }
   if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
   hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
                      method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);

@@ -18,6 +18,29 @@ namespace seemps {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Arithmetic of the product: FP64 MFMA, or FP32 MFMA with FP32 accumulation
+// (operands rounded to FP32 on their way into LDS; C stays complex128).
+// Output row of accumulator item r on a lane with k-group lk: f64 16x16x4 gives
+// D[lk + 4r][l%16], f32 16x16x4 gives D[4 lk + r][l%16] (both checked on the
+// device); the operand layouts are the same.
+struct PrecF64 {
+  typedef double T;
+  typedef d4 V;
+  static __device__ __forceinline__ V mfma(T a, T b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+struct PrecF32 {
+  typedef float T;
+  typedef f4 V;
+  static __device__ __forceinline__ V mfma(T a, T b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
 
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
 constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
@@ -47,8 +70,8 @@ __device__ __forceinline__ void load_a(const double2* __restrict__ A, int64_t ld
   }
 }
 
-template <int OP>
-__device__ __forceinline__ void store_a(double* __restrict__ s_re, double* __restrict__ s_im,
+template <int OP, typename T>
+__device__ __forceinline__ void store_a(T* __restrict__ s_re, T* __restrict__ s_im,
                                         const double2 (&r)[4]) {
   const int t = threadIdx.x;
   const double sg = (OP == SEEMPS_OP_C || OP == SEEMPS_OP_H) ? -1.0 : 1.0;
@@ -56,15 +79,15 @@ __device__ __forceinline__ void store_a(double* __restrict__ s_re, double* __res
     const int m = t >> 2, kb = (t & 3) * 4;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s_re[(kb + u) * LDA_S + m] = r[u].x;
-      s_im[(kb + u) * LDA_S + m] = sg * r[u].y;
+      s_re[(kb + u) * LDA_S + m] = (T)r[u].x;
+      s_im[(kb + u) * LDA_S + m] = (T)(sg * r[u].y);
     }
   } else {
     const int k = t >> 4, mb = (t & 15) * 4;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s_re[k * LDA_S + mb + u] = r[u].x;
-      s_im[k * LDA_S + mb + u] = sg * r[u].y;
+      s_re[k * LDA_S + mb + u] = (T)r[u].x;
+      s_im[k * LDA_S + mb + u] = (T)(sg * r[u].y);
     }
   }
 }
@@ -94,8 +117,8 @@ __device__ __forceinline__ void load_b(const double2* __restrict__ B, int64_t ld
   }
 }
 
-template <int OP>
-__device__ __forceinline__ void store_b(double* __restrict__ s_re, double* __restrict__ s_im,
+template <int OP, typename T>
+__device__ __forceinline__ void store_b(T* __restrict__ s_re, T* __restrict__ s_im,
                                         const double2 (&r)[4]) {
   const int t = threadIdx.x;
   const double sg = (OP == SEEMPS_OP_C || OP == SEEMPS_OP_H) ? -1.0 : 1.0;
@@ -103,28 +126,30 @@ __device__ __forceinline__ void store_b(double* __restrict__ s_re, double* __res
     const int k = t >> 4, nb = (t & 15) * 4;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s_re[k * LDB_S + nb + u] = r[u].x;
-      s_im[k * LDB_S + nb + u] = sg * r[u].y;
+      s_re[k * LDB_S + nb + u] = (T)r[u].x;
+      s_im[k * LDB_S + nb + u] = (T)(sg * r[u].y);
     }
   } else {
     const int n = t >> 2, kb = (t & 3) * 4;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      s_re[(kb + u) * LDB_S + n] = r[u].x;
-      s_im[(kb + u) * LDB_S + n] = sg * r[u].y;
+      s_re[(kb + u) * LDB_S + n] = (T)r[u].x;
+      s_im[(kb + u) * LDB_S + n] = (T)(sg * r[u].y);
     }
   }
 }
 
 // Batch index z uses K (the last batch uses Klast: split-K slices share one
 // launch, the final slice being shorter).
-template <int OPA, int OPB>
+template <int OPA, int OPB, typename P>
 __global__ __launch_bounds__(256) void zgemm_kernel(
     int64_t M, int64_t N, int64_t Kfull, int64_t Klast, double2 alpha,
     const double2* __restrict__ A, int64_t lda, int64_t sA, const double2* __restrict__ B,
     int64_t ldb, int64_t sB, double2 beta, double2* __restrict__ C, int64_t ldc, int64_t sC) {
-  __shared__ double as_re[BK * LDA_S], as_im[BK * LDA_S];
-  __shared__ double bs_re[BK * LDB_S], bs_im[BK * LDB_S];
+  typedef typename P::T T;
+  typedef typename P::V V;
+  __shared__ T as_re[BK * LDA_S], as_im[BK * LDA_S];
+  __shared__ T bs_re[BK * LDB_S], bs_im[BK * LDB_S];
 
   const int64_t bz = blockIdx.z;
   const int64_t K = (bz == (int64_t)gridDim.z - 1) ? Klast : Kfull;
@@ -138,21 +163,21 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
 
-  d4 cr[2][2], ci[2][2];
+  V cr[2][2], ci[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      cr[a][b] = V{0, 0, 0, 0};
+      ci[a][b] = V{0, 0, 0, 0};
     }
 
   double2 ra[4], rb[4];
   load_a<OPA>(A, lda, M, K, m0, 0, ra);
   load_b<OPB>(B, ldb, K, N, 0, n0, rb);
   for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    store_a<OPA>(as_re, as_im, ra);
-    store_b<OPB>(bs_re, bs_im, rb);
+    store_a<OPA, T>(as_re, as_im, ra);
+    store_b<OPB, T>(bs_re, bs_im, rb);
     __syncthreads();
     if (k0 + BK < K) {
       load_a<OPA>(A, lda, M, K, m0, k0 + BK, ra);
@@ -161,7 +186,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int k = ks * 4 + lk;
-      double ar[2], ai[2], br[2], bi[2];
+      T ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         ar[a] = as_re[k * LDA_S + wm + a * 16 + li];
@@ -176,10 +201,10 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[a], bi[b], cr[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+          cr[a][b] = P::mfma(ar[a], br[b], cr[a][b]);
+          cr[a][b] = P::mfma(-ai[a], bi[b], cr[a][b]);
+          ci[a][b] = P::mfma(ar[a], bi[b], ci[a][b]);
+          ci[a][b] = P::mfma(ai[a], br[b], ci[a][b]);
         }
     }
     __syncthreads();
@@ -192,10 +217,10 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t m = m0 + wm + a * 16 + lk + 4 * r;
+        const int64_t m = m0 + wm + a * 16 + P::row(lk, r);
         const int64_t n = n0 + wn + b * 16 + li;
         if (m < M && n < N) {
-          const double2 acc = make_double2(cr[a][b][r], ci[a][b][r]);
+          const double2 acc = make_double2((double)cr[a][b][r], (double)ci[a][b][r]);
           double2 out = cmul(alpha, acc);
           if (use_beta) out = cfma(beta, C[m * ldc + n], out);
           C[m * ldc + n] = out;
@@ -204,11 +229,15 @@ __global__ __launch_bounds__(256) void zgemm_kernel(
 }
 
 template <int OPA, int OPB>
-void launch(dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K, int64_t Klast,
+void launch(int prec, dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K, int64_t Klast,
             double2 alpha, const double2* A, int64_t lda, int64_t sA, const double2* B,
             int64_t ldb, int64_t sB, double2 beta, double2* C, int64_t ldc, int64_t sC) {
-  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), grid, dim3(256), 0, s, M, N, K, Klast, alpha, A,
-                     lda, sA, B, ldb, sB, beta, C, ldc, sC);
+  if (prec == SEEMPS_PREC_FP32)
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF32>), grid, dim3(256), 0, s, M, N, K, Klast,
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+  else
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64>), grid, dim3(256), 0, s, M, N, K, Klast,
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
 }
 
 // C[m, n] = alpha * sum_z P[z][m, n] + beta * C[m, n]  (P packed M x N slices).
@@ -228,16 +257,17 @@ __global__ void splitk_reduce(int64_t M, int64_t N, int nz, const double2* __res
 }
 
 // flops: algorithmic count of the whole product (for kernel timing).
-int dispatch(int opA, int opB, dim3 grid, hipStream_t stream, int64_t M, int64_t N, int64_t K,
-             int64_t Klast, double2 alpha, const double2* a, int64_t lda, int64_t sA,
+int dispatch(int prec, int opA, int opB, dim3 grid, hipStream_t stream, int64_t M, int64_t N,
+             int64_t K, int64_t Klast, double2 alpha, const double2* a, int64_t lda, int64_t sA,
              const double2* b, int64_t ldb, int64_t sB, double2 beta, double2* c, int64_t ldc,
              int64_t sC, double flops) {
-  TimedLaunch timed("zgemm_kernel", stream, flops);
-#define SEEMPS_GEMM_CASE(x, y)                                                               \
-  if (opA == x && opB == y) {                                                                \
-    launch<x, y>(grid, stream, M, N, K, Klast, alpha, a, lda, sA, b, ldb, sB, beta, c, ldc, \
-                 sC);                                                                        \
-    return check_launch("zgemm");                                                            \
+  TimedLaunch timed(prec == SEEMPS_PREC_FP32 ? "zgemm_kernel_fp32" : "zgemm_kernel", stream,
+                    flops);
+#define SEEMPS_GEMM_CASE(x, y)                                                              \
+  if (opA == x && opB == y) {                                                               \
+    launch<x, y>(prec, grid, stream, M, N, K, Klast, alpha, a, lda, sA, b, ldb, sB, beta, c, \
+                 ldc, sC);                                                                  \
+    return check_launch("zgemm");                                                           \
   }
   SEEMPS_GEMM_CASE(0, 0) SEEMPS_GEMM_CASE(0, 1) SEEMPS_GEMM_CASE(0, 2) SEEMPS_GEMM_CASE(0, 3)
   SEEMPS_GEMM_CASE(1, 0) SEEMPS_GEMM_CASE(1, 1) SEEMPS_GEMM_CASE(1, 2) SEEMPS_GEMM_CASE(1, 3)
@@ -269,10 +299,11 @@ int64_t splitk_workspace(int64_t M, int64_t N, int64_t K) {
 // tiles to fill the device (deterministic: slices reduced in a fixed order).
 int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
              int64_t lda, const void* B, int64_t ldb, double2 beta, void* C, int64_t ldc,
-             double2* ws, int64_t ws_elems, hipStream_t stream) {
+             double2* ws, int64_t ws_elems, hipStream_t stream, int prec) {
   const int64_t z = splitk_slices(M, N, K);
   if (z <= 1 || ws == nullptr || ws_elems < z * M * N)
-    return zgemm(opA, opB, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, stream);
+    return zgemm(opA, opB, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, stream,
+                 prec);
   SEEMPS_CHECK_ARG(ceil_div(M, BM) <= 65535, "zgemm: M too large");
   int64_t Kc = ceil_div(ceil_div(K, z), BK) * BK;
   const int64_t nz = ceil_div(K, Kc);
@@ -280,7 +311,7 @@ int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, c
   const int64_t sA = (opA == SEEMPS_OP_N || opA == SEEMPS_OP_C) ? Kc : Kc * lda;
   const int64_t sB = (opB == SEEMPS_OP_N || opB == SEEMPS_OP_C) ? Kc * ldb : Kc;
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)nz);
-  int st = dispatch(opA, opB, grid, stream, M, N, Kc, Klast, make_double2(1.0, 0.0),
+  int st = dispatch(prec, opA, opB, grid, stream, M, N, Kc, Klast, make_double2(1.0, 0.0),
                     (const double2*)A, lda, sA, (const double2*)B, ldb, sB,
                     make_double2(0.0, 0.0), ws, N, M * N, 8.0 * M * N * K);
   if (st) return st;
@@ -295,8 +326,10 @@ int zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, c
 // Enqueue a complex GEMM (shared by the other host entry points of the library).
 int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, const void* A,
           int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB, double2 beta, void* C,
-          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream) {
+          int64_t ldc, int64_t sC, int64_t batch, hipStream_t stream, int prec) {
   SEEMPS_CHECK_ARG(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "zgemm: invalid op flags");
+  SEEMPS_CHECK_ARG(prec == SEEMPS_PREC_FP64 || prec == SEEMPS_PREC_FP32,
+                   "zgemm: invalid precision");
   SEEMPS_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative dimension");
   if (M == 0 || N == 0 || batch == 0) return SEEMPS_OK;
   SEEMPS_CHECK_ARG(batch <= 65535, "zgemm: batch too large");
@@ -308,7 +341,7 @@ int zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, double2 alpha, cons
                        stream);
   }
   dim3 grid((unsigned)ceil_div(N, BN), (unsigned)ceil_div(M, BM), (unsigned)batch);
-  return dispatch(opA, opB, grid, stream, M, N, K, K, alpha, (const double2*)A, lda, sA,
+  return dispatch(prec, opA, opB, grid, stream, M, N, K, K, alpha, (const double2*)A, lda, sA,
                   (const double2*)B, ldb, sB, beta, (double2*)C, ldc, sC,
                   8.0 * M * N * K * batch);
 }
@@ -323,4 +356,14 @@ extern "C" int seemps_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K, d
   return seemps::zgemm(opA, opB, M, N, K, make_double2(alpha_re, alpha_im), A, lda, strideA, B,
                        ldb, strideB, make_double2(beta_re, beta_im), C, ldc, strideC, batch,
                        (hipStream_t)stream);
+}
+
+extern "C" int seemps_zgemm_prec(int precision, int opA, int opB, int64_t M, int64_t N,
+                                 int64_t K, double alpha_re, double alpha_im, const void* A,
+                                 int64_t lda, int64_t strideA, const void* B, int64_t ldb,
+                                 int64_t strideB, double beta_re, double beta_im, void* C,
+                                 int64_t ldc, int64_t strideC, int64_t batch, void* stream) {
+  return seemps::zgemm(opA, opB, M, N, K, make_double2(alpha_re, alpha_im), A, lda, strideA, B,
+                       ldb, strideB, make_double2(beta_re, beta_im), C, ldc, strideC, batch,
+                       (hipStream_t)stream, precision);
 }

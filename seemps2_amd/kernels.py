@@ -52,9 +52,13 @@ def fill_zero(t: torch.Tensor) -> torch.Tensor:
     return t
 
 
+PRECISIONS = {"fp64": 0, "fp32": 1, "mixed": 2}
+
+
 def gemm(A: torch.Tensor, B: torch.Tensor, opA: int = 0, opB: int = 0, alpha=1.0,
-         beta=0.0, C: torch.Tensor | None = None) -> torch.Tensor:
-    """C = alpha op(A) op(B) + beta C for 2-D contiguous complex128 tensors."""
+         beta=0.0, C: torch.Tensor | None = None, precision: str = "fp64") -> torch.Tensor:
+    """C = alpha op(A) op(B) + beta C for 2-D contiguous complex128 tensors
+    (``precision`` "fp32": FP32 products and accumulation)."""
     if A.dim() != 2 or B.dim() != 2:
         raise ValueError("gemm expects matrices")
     M = A.shape[1] if opA in (1, 2) else A.shape[0]
@@ -67,9 +71,9 @@ def gemm(A: torch.Tensor, B: torch.Tensor, opA: int = 0, opB: int = 0, alpha=1.0
         C = empty(M, N)
         beta = 0.0
     alpha, beta = complex(alpha), complex(beta)
-    check(lib.seemps_zgemm(opA, opB, M, N, K, alpha.real, alpha.imag, ptr(A), A.shape[1], 0,
-                           ptr(B), B.shape[1], 0, beta.real, beta.imag, ptr(C), N, 0, 1,
-                           stream()))
+    check(lib.seemps_zgemm_prec(PRECISIONS[precision], opA, opB, M, N, K, alpha.real,
+                                alpha.imag, ptr(A), A.shape[1], 0, ptr(B), B.shape[1], 0,
+                                beta.real, beta.imag, ptr(C), N, 0, 1, stream()))
     return C
 
 
@@ -240,16 +244,21 @@ def two_site_matvec(L: torch.Tensor, Hm: torch.Tensor, R: torch.Tensor, v: torch
 
 
 def eigensolve(L: torch.Tensor, Hm: torch.Tensor, R: torch.Tensor, v0: torch.Tensor,
-               d1: int, d2: int, ncv: int = 20, tol: float = 0.0, maxiter: int = 1000):
-    """Lowest eigenpair of the two-site operator; returns (eigenvalue, vector, stats)."""
+               d1: int, d2: int, ncv: int = 20, tol: float = 0.0, maxiter: int = 1000,
+               precision: str = "fp64"):
+    """Lowest eigenpair of the two-site operator; returns (eigenvalue, vector, stats).
+
+    ``precision`` selects the arithmetic of the operator's contractions: "fp64",
+    "fp32", or "mixed" (FP32 restarts down to a 1e-5 relative residual, FP64 to
+    ``tol``)."""
     chiL, c, _ = L.shape
     chiR, w2, _ = R.shape
     evec = empty(chiL, d1, d2, chiR)
-    stats = (ctypes.c_double * 4)()
+    stats = (ctypes.c_double * 5)()
     wb = lib.seemps_eigensolve_workspace(chiL, c, d1, d2, w2, chiR, ncv)
     wp, wn = workspace.get(wb)
-    check(lib.seemps_two_site_eigensolve(chiL, c, d1, d2, w2, chiR, ptr(L), ptr(Hm), ptr(R),
-                                         ptr(v0), ptr(evec), ncv, tol, maxiter, stats, wp, wn,
-                                         stream()))
+    check(lib.seemps_two_site_eigensolve_prec(PRECISIONS[precision], chiL, c, d1, d2, w2, chiR,
+                                              ptr(L), ptr(Hm), ptr(R), ptr(v0), ptr(evec), ncv,
+                                              tol, maxiter, stats, wp, wn, stream()))
     return stats[0], evec, {"residual": stats[1], "matvecs": int(stats[2]),
-                            "restarts": int(stats[3])}
+                            "restarts": int(stats[3]), "matvecs_fp32": int(stats[4])}

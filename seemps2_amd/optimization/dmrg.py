@@ -23,6 +23,10 @@ from ..operators.quadratic import QuadraticForm
 from ..state import MPS, CanonicalMPS, random_mps
 
 NCV = 20
+# Arithmetic of the local eigensolver's contractions: "fp64" (the reference's),
+# or "mixed" (FP32 products until the residual reaches 1e-5 of the eigenvalue,
+# FP64 from there to tol_eigs; same stopping test as "fp64").
+PRECISIONS = ("fp64", "mixed")
 
 
 @dataclasses.dataclass
@@ -43,6 +47,7 @@ class SweepStats:
     def __init__(self):
         self.bond_updates = 0
         self.matvecs = 0
+        self.matvecs_fp32 = 0
         self.restarts = 0
 
     def reset(self):
@@ -52,26 +57,30 @@ class SweepStats:
 STATS = SweepStats()
 
 
-def _diagonalize_two_site(QF: QuadraticForm, i: int, tol_eigs: float, ncv: int = NCV):
+def _diagonalize_two_site(QF: QuadraticForm, i: int, tol_eigs: float, ncv: int = NCV,
+                          precision: str = "fp64"):
     """Lowest eigenpair of the two-site effective Hamiltonian (dmrg.py:15-23)."""
     L, Hm, R = QF.two_site_block(i)
     v = _contract_last_and_first(QF.state[i], QF.state[i + 1])
     d1 = QF.O[i].shape[1]
     d2 = QF.O[i + 1].shape[1]
-    eval_, evec, stats = K.eigensolve(L, Hm, R, v, d1, d2, ncv=ncv, tol=tol_eigs)
+    eval_, evec, stats = K.eigensolve(L, Hm, R, v, d1, d2, ncv=ncv, tol=tol_eigs,
+                                      precision=precision)
     STATS.matvecs += stats["matvecs"]
+    STATS.matvecs_fp32 += stats["matvecs_fp32"]
     STATS.restarts += stats["restarts"]
     STATS.bond_updates += 1
     return eval_, evec
 
 
-def _sweep(QF: QuadraticForm, direction: int, tol_eigs: float, strategy: Strategy) -> float:
+def _sweep(QF: QuadraticForm, direction: int, tol_eigs: float, strategy: Strategy,
+           precision: str = "fp64") -> float:
     """One two-site sweep updating QF in place (dmrg.py:26-42)."""
     size = QF.state.size
     sites = range(size - 1) if direction > 0 else range(size - 2, -1, -1)
     E = np.inf
     for i in sites:
-        E, AB = _diagonalize_two_site(QF, i, tol_eigs)
+        E, AB = _diagonalize_two_site(QF, i, tol_eigs, precision=precision)
         if direction > 0:
             QF.update_2site_right(AB, i, strategy)
         else:
@@ -107,11 +116,16 @@ def _state_deepcopy(state: MPS) -> MPS:
 
 def dmrg(H, guess: MPS | None = None, maxiter: int = 20, tol: float = 1e-10,
          tol_up: float | None = None, tol_eigs: float | None = None,
-         strategy: Strategy = DEFAULT_STRATEGY, callback: Callable | None = None
-         ) -> OptimizeResults:
-    """Ground state of an MPO (or NNHamiltonian) by two-site DMRG (dmrg.py:76-199)."""
+         strategy: Strategy = DEFAULT_STRATEGY, callback: Callable | None = None,
+         precision: str = "fp64") -> OptimizeResults:
+    """Ground state of an MPO (or NNHamiltonian) by two-site DMRG (dmrg.py:76-199).
+
+    ``precision`` (not in the reference) selects the local eigensolver's
+    arithmetic: "fp64", or "mixed" (see ``PRECISIONS``)."""
     if maxiter < 1:
         raise ValueError("maxiter must be positive")
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
     if isinstance(H, NNHamiltonian):
         H = H.to_mpo()
     if H.size < 2:
@@ -144,7 +158,7 @@ def dmrg(H, guess: MPS | None = None, maxiter: int = 20, tol: float = 1e-10,
         callback(QF.state, results)
     last_energy = energy
     for _sweep_no in range(1, maxiter + 1):
-        _sweep(QF, direction, tol_eigs, strat)
+        _sweep(QF, direction, tol_eigs, strat, precision)
         direction = -direction
         energy, variance = _energy_and_variance(H, QF.state)
         results.trajectory.append(energy)

@@ -156,7 +156,7 @@ def two_site_matvec(L, Hm, R, v, d1, d2):
     return _t(oa.two_site_matvec(_np(L), _h12(Hm, c, d1, d2, w2), _np(R), _np(v)))
 
 
-def eigensolve(L, Hm, R, v0, d1, d2, ncv=20, tol=0.0, maxiter=1000):
+def eigensolve(L, Hm, R, v0, d1, d2, ncv=20, tol=0.0, maxiter=1000, precision="fp64"):
     c, w2 = L.shape[1], R.shape[1]
     H12 = _h12(Hm, c, d1, d2, w2)
     shape = (L.shape[0], d1, d2, R.shape[0])
@@ -168,4 +168,4 @@ def eigensolve(L, Hm, R, v0, d1, d2, ncv=20, tol=0.0, maxiter=1000):
         Hd[:, k] = oa.two_site_matvec(_np(L), H12, _np(R), e.reshape(shape)).reshape(-1)
     w, V = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
     return float(w[0]), _t(V[:, 0].reshape(shape)), {"residual": 0.0, "matvecs": n,
-                                                     "restarts": 1}
+                                                     "restarts": 1, "matvecs_fp32": 0}

@@ -159,3 +159,26 @@ def test_dmrg_free_fermions_gpu(S):
     out = S.dmrg(S.XXZHamiltonian(N, 1.0, 0.0), dev_mps(S, g), maxiter=12,
                  strategy=S.Strategy(max_bond_dimension=128))
     assert abs(out.energy - ex.xx_free_fermion_energy(N)) < 1e-9
+
+
+def test_dmrg_mixed_precision_matches_fp64(S):
+    """FP32 contractions in the early Lanczos restarts leave the DMRG energy
+    within 1e-9 (relative) of the FP64 run and of the exact free-fermion energy."""
+    import importlib
+
+    # the package re-exports the dmrg function under the module's name
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
+
+    N = 16
+    rng = np.random.default_rng(11)
+    g = ob.random_mps([2] * N, D=8, complex=True, rng=rng)
+    st = S.Strategy(max_bond_dimension=64)
+    H = S.XXZHamiltonian(N, 1.0, 0.0)
+    E64 = S.dmrg(H, dev_mps(S, g), maxiter=10, strategy=st).energy
+    D.STATS.reset()
+    out = S.dmrg(H, dev_mps(S, g), maxiter=10, strategy=st, precision="mixed")
+    assert D.STATS.matvecs_fp32 > 0
+    assert abs(out.energy - E64) <= 1e-9 * abs(E64)
+    assert abs(out.energy - ex.xx_free_fermion_energy(N)) < 1e-9
+    with pytest.raises(ValueError):
+        S.dmrg(H, dev_mps(S, g), maxiter=1, strategy=st, precision="fp16")

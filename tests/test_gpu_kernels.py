@@ -42,6 +42,38 @@ def test_zgemm_ops(K, rng, opA, opB, M, N, K_):
     assert rel(K.to_numpy(out), ref) < 1e-14
 
 
+@pytest.mark.parametrize("opA", [0, 1, 2, 3])
+@pytest.mark.parametrize("opB", [0, 3, 1])
+@pytest.mark.parametrize("M,N,K_", [(37, 53, 29), (130, 70, 200)])
+def test_zgemm_fp32_exact_on_small_integers(K, rng, opA, opB, M, N, K_):
+    """Integer operands in [-4, 4]: every product and partial sum is exact in
+    FP32, so the FP32 path must equal the exact result (pins the FP32 MFMA
+    output layout independently of rounding)."""
+    def ints(*shape):
+        return rng.integers(-4, 5, size=shape) + 1j * rng.integers(-4, 5, size=shape)
+
+    A = ints(*((K_, M) if opA in (1, 2) else (M, K_)))
+    B = ints(*((N, K_) if opB in (1, 2) else (K_, N)))
+    C = ints(M, N)
+    ref = OPS[opA](A) @ OPS[opB](B) - C
+    out = K.gemm(K.as_dev(A), K.as_dev(B), opA, opB, 1.0, -1.0, K.as_dev(C), precision="fp32")
+    assert np.array_equal(K.to_numpy(out), ref)
+
+
+@pytest.mark.parametrize("M,N,K_", [(1280, 1024, 256), (300, 77, 1000)])
+def test_zgemm_fp32_rounding(K, rng, M, N, K_):
+    """Gaussian operands: the FP32 product's error stays at FP32 rounding
+    (normwise relative <= sqrt(K) 2^-24 up to a small factor; 4e-7 measured at
+    K = 256), far above FP64's."""
+    A = crandn(rng, M, K_)
+    B = crandn(rng, K_, N)
+    ref = A @ B
+    out = K.to_numpy(K.gemm(K.as_dev(A), K.as_dev(B), precision="fp32"))
+    err = rel(out, ref)
+    assert err < 2e-6 * np.sqrt(K_ / 256)
+    assert err > 1e-12  # really computed in FP32
+
+
 def test_zlocal(K, rng):
     for (P, I, Q, J1, J2) in [(7, 20, 130, 20, 1), (33, 2, 40, 10, 5), (1, 1, 1, 1, 1), (5, 10, 64, 10, 1)]:
         x = crandn(rng, P, I, Q)
@@ -180,6 +212,69 @@ def test_two_site_matvec_and_eigensolve(K, rng):
     assert abs(lam - w[0]) < 1e-9 * max(1, abs(w[0]))
     x = K.to_numpy(vec).reshape(-1)
     assert abs(abs(np.vdot(V[:, 0], x)) - 1) < 1e-8
+    # mixed precision: from a random start the first cycle runs FP32 products
+    # after its first (FP64) one; the same FP64 stopping test at the end
+    lam_m, vec_m, st_m = K.eigensolve(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d, ncv=20,
+                                      tol=1e-12, precision="mixed")
+    assert st_m["matvecs_fp32"] == 19
+    assert st_m["matvecs"] > 20
+    assert abs(lam_m - w[0]) < 1e-9 * max(1, abs(w[0]))
+    assert abs(abs(np.vdot(V[:, 0], K.to_numpy(vec_m).reshape(-1))) - 1) < 1e-8
+    # FP32 throughout: eigenvalue to FP32 accuracy only
+    lam_s, _, st_s = K.eigensolve(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d, ncv=20,
+                                  tol=1e-4, precision="fp32")
+    assert st_s["matvecs_fp32"] == st_s["matvecs"]
+    assert abs(lam_s - w[0]) < 1e-5 * max(1, abs(w[0]))
+    # started from the eigenvector, mixed stays in FP64
+    _, _, st_e = K.eigensolve(K.as_dev(L), Hm, K.as_dev(R), vec, d, d, ncv=20, tol=1e-12,
+                              precision="mixed")
+    assert st_e["matvecs_fp32"] == 0
+
+
+def test_eigensolve_early_exit_in_cycle(K, rng):
+    """Operators of dimension >= 8192 test the Ritz residual inside a cycle: a
+    warm start stops after fewer than ncv products, at the same accuracy as the
+    dense eigensolver."""
+    from oracle import builders as obl
+
+    H = obl.xxz_mpo(6)
+    O1, O2 = H[2], H[3]
+    c, w2, d = O1.shape[0], O2.shape[3], 2
+    chi = 48  # n = 48 * 4 * 48 = 9216
+    # Hermitian environments: L[a, c, b] = conj(L[b, c, a]) (R alike)
+    L = crandn(rng, chi, c, chi)
+    L = 0.5 * (L + L.conj().transpose(2, 1, 0))
+    R = crandn(rng, chi, w2, chi)
+    R = 0.5 * (R + R.conj().transpose(2, 1, 0))
+    Hm = K.two_site_operator(K.as_dev(O1), K.as_dev(O2))
+    v = crandn(rng, chi, d, d, chi)
+    Ld, Rd = K.as_dev(L), K.as_dev(R)
+    lam0, vec0, st0 = K.eigensolve(Ld, Hm, Rd, K.as_dev(v), d, d, ncv=20, tol=1e-12,
+                                   maxiter=400)
+    # the reference value: scipy's eigsh on the same operator (CPU)
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    from oracle import algorithms as oal
+
+    H12 = np.tensordot(O1, O2, axes=1)
+    n = v.size
+    op = LinearOperator((n, n), dtype=complex,
+                        matvec=lambda x: oal.two_site_matvec(L, H12, R, x.reshape(v.shape)).ravel())
+    ref = eigsh(op, k=1, which="SA", tol=1e-13, ncv=40, maxiter=10000)[0][0]
+    assert abs(lam0 - ref) < 1e-9 * abs(ref)
+    # started from the converged vector: the first in-cycle test (after 8
+    # products) stops it (random environments leave a tiny spectral gap, so
+    # only a start this close converges within one cycle)
+    lam1, _, st1 = K.eigensolve(Ld, Hm, Rd, vec0, d, d, ncv=20, tol=1e-10)
+    assert st1["restarts"] == 1 and st1["matvecs"] == 8
+    assert abs(lam1 - ref) < 1e-9 * abs(ref)
+    # below the size threshold the same start runs the whole cycle
+    small = slice(0, 40)
+    Ls, Rs = L[small, :, small], R[small, :, small]  # n = 40 * 4 * 40 = 6400
+    lam_s, vec_s, _ = K.eigensolve(K.as_dev(Ls), Hm, K.as_dev(Rs),
+                                   K.as_dev(v[small, :, :, small]), d, d, ncv=20, tol=1e-12,
+                                   maxiter=400)
+    _, _, st_s = K.eigensolve(K.as_dev(Ls), Hm, K.as_dev(Rs), vec_s, d, d, ncv=20, tol=1e-10)
+    assert st_s["matvecs"] == 20
 
 
 @pytest.mark.parametrize("chiL,chiR", [(64, 64), (96, 40), (256, 256)])
