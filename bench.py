@@ -219,12 +219,14 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     rel = abs(E - E_plain) / max(abs(E_plain), 1e-300)
     log(f"learned form: {n_chi} chi={chi} updates in {t_chi:.3f} s ({value:.2f}/s, "
         f"x{value / max(value_plain, 1e-30):.2f}), energy rel. diff {rel:.2e}, "
-        f"{lt.fallbacks} fallbacks of {lt.calls} splits, {mv32}/{matvecs} FP32 matvecs, "
+        f"{lt.fallbacks} fallbacks of {lt.calls} splits ({lt.basis_splits} basis-sketched), "
+        f"{mv32}/{matvecs} FP32 matvecs, "
         f"training {t_train:.2f} s "
         f"(loss {losses[0]:.2e} -> {losses[-1]:.2e} on {X.shape[0]} spectra)")
     return dict(value=round(value, 3), unit="bond_updates/s",
                 speedup_vs_plain=round(value / max(value_plain, 1e-30), 3),
                 energy_rel_diff=float(f"{rel:.3e}"), fallbacks=lt.fallbacks, splits=lt.calls,
+                basis_splits=lt.basis_splits, full_splits=lt.full_splits,
                 mean_sketch_width=round(lt.sketch_columns / max(lt.calls - lt.full_splits, 1), 1),
                 power_iters=power_iters, eigensolver_precision=precision,
                 matvecs=matvecs, matvecs_fp32=mv32,

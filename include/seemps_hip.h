@@ -241,6 +241,26 @@ int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A, int mode, int m
                             int64_t* D_host, double* err_host, int32_t* fallback_host,
                             void* work, int64_t work_bytes, int32_t* info, void* stream);
 
+/* Split sketched by an isometry of the bond (no counterpart in the reference:
+ * the learned-rank truncation).  side 0: basis is k x n with orthonormal rows
+ * (the right-canonical neighbour a sweep to the right replaces); side 1: m x k
+ * with orthonormal columns (the left-canonical neighbour of a sweep to the
+ * left).  Omega = [basis^H | p expansion directions] (side 0; side 1 alike on
+ * A^H), the expansion directions being Z = X^H X G (G complex Gaussian from
+ * `seed`, X = A or A^H) projected off the basis and orthonormalised (twice
+ * each); W = X Omega; one-sided Jacobi SVD of W without QR preconditioning;
+ * truncation on W's k + p singular values with ||A||_F^2 - ||W||_F^2 counted
+ * as discarded, and the same fallback to seemps_svd_split as
+ * seemps_svd_split_sketch (also when k + p > min(m, n)).  Outputs as
+ * seemps_svd_split (s_out receives k + p values). */
+int64_t seemps_svd_split_basis_workspace(int64_t m, int64_t n, int side, int64_t k, int64_t p);
+int seemps_svd_split_basis(int64_t m, int64_t n, const void* A, int side, const void* basis,
+                           int64_t k, int64_t p, uint64_t seed, int mode, int method,
+                           double tolerance, int64_t max_bond, void* outA, void* outB,
+                           double* s_out, int64_t* D_host, double* err_host,
+                           int32_t* fallback_host, void* work, int64_t work_bytes,
+                           int32_t* info, void* stream);
+
 /* Features of the learned rank predictor for an m x n matrix A (row-major,
  * lda = n): 32 sketch columns Y = A Omega (Omega from `seed`), eigenvalues of
  * Y^H Y, features[0] = log10(min(m, n)) / 4 and features[i] =

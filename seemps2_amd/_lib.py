@@ -80,6 +80,12 @@ _SIGNATURES = {
         [_I, _I, _P, c_int, c_int, _D, _I, _I, _I, ctypes.c_uint64, _P, _P, _P,
          POINTER(c_int64), POINTER(c_double), POINTER(c_int32), _P, _I, POINTER(c_int32), _P],
     ),
+    "seemps_svd_split_basis_workspace": (_I, [_I, _I, c_int, _I, _I]),
+    "seemps_svd_split_basis": (
+        c_int,
+        [_I, _I, _P, c_int, _P, _I, _I, ctypes.c_uint64, c_int, c_int, _D, _I, _P, _P, _P,
+         POINTER(c_int64), POINTER(c_double), POINTER(c_int32), _P, _I, POINTER(c_int32), _P],
+    ),
     "seemps_spectral_sketch_workspace": (_I, [_I, _I]),
     "seemps_spectral_sketch": (c_int, [_I, _I, _P, ctypes.c_uint64, _P, _P, _I, _P]),
     "seemps_mlp_param_count": (_I, [c_int, c_int]),

@@ -311,23 +311,29 @@ def destructively_truncate_vector(s, strategy: Strategy) -> float:
     return err
 
 
-def _split(AA, strategy: Strategy, mode: int):
+def _split(AA, strategy: Strategy, mode: int, basis=None):
     from .learned import two_site_split
 
     AA = K.as_dev(AA)
     a, d1, d2, b = AA.shape
-    A2, B2, D, err2 = two_site_split(AA, a * d1, d2 * b, mode, strategy)
+    A2, B2, D, err2 = two_site_split(AA, a * d1, d2 * b, mode, strategy, basis)
     return A2.view(a, d1, D), B2.view(D, d2, b), math.sqrt(err2)
 
 
-def _left_orth_2site(AA, strategy: Strategy):
-    """AA[a,i,j,b] -> (U[a,i,D] isometry, sV[D,j,b], sqrt(err)) (schmidt.pxi:123-147)."""
-    return _split(AA, strategy, 0)
+def _left_orth_2site(AA, strategy: Strategy, basis=None):
+    """AA[a,i,j,b] -> (U[a,i,D] isometry, sV[D,j,b], sqrt(err)) (schmidt.pxi:123-147).
+
+    ``basis`` (optional): the current right neighbour B[k,j,b] when it is
+    right-canonical; only the learned-rank split uses it, as its sketch."""
+    return _split(AA, strategy, 0, basis)
 
 
-def _right_orth_2site(AA, strategy: Strategy):
-    """AA[a,i,j,b] -> (Us[a,i,D], V[D,j,b] isometry, sqrt(err)) (schmidt.pxi:150-174)."""
-    return _split(AA, strategy, 1)
+def _right_orth_2site(AA, strategy: Strategy, basis=None):
+    """AA[a,i,j,b] -> (Us[a,i,D], V[D,j,b] isometry, sqrt(err)) (schmidt.pxi:150-174).
+
+    ``basis`` (optional): the current left neighbour A[a,i,k] when it is
+    left-canonical; only the learned-rank split uses it, as its sketch."""
+    return _split(AA, strategy, 1, basis)
 
 
 def _move_right(state: list, A, site: int, strategy: Strategy) -> float:

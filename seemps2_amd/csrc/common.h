@@ -115,12 +115,15 @@ int zmdotc_impl(int64_t n, int64_t nv, const double2* V, int64_t ldv, const doub
                 double2* result, double2* work, hipStream_t stream);
 
 // SVD and QR building blocks (svd.hip, qr.hip).
+// precondition = false skips the QR preconditioner (for matrices whose columns
+// are already nearly orthogonal); the workspace query, the SVD and the
+// completion must then all be called with the same flag.
 int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
                 double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
-                int* info_dev_off, bool complete = true);
+                int* info_dev_off, bool complete = true, bool precondition = true);
 int svd_complete_enqueue(int64_t m, int64_t n, double2* U, double2* VH, void* work,
-                         const int64_t* kmax_dev, hipStream_t stream);
-int64_t svd_workspace_bytes(int64_t m, int64_t n);
+                         const int64_t* kmax_dev, hipStream_t stream, bool precondition = true);
+int64_t svd_workspace_bytes(int64_t m, int64_t n, bool precondition = true);
 int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t max_bond,
                      int64_t* D_out, double* err_out, hipStream_t stream);
 int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream);

@@ -54,16 +54,18 @@ __global__ void identity_cm(int64_t len, int64_t ncol, double2* __restrict__ out
 
 // Truncation on the sketch spectrum s[0..l) with the unsketched weight
 // frob - sum s^2 counted as discarded (see the file comment).  full = min(m, n).
+// resid2 = ||A - Q Q^H A||_F^2 computed from the residual itself: the
+// difference ||A||^2 - sum s^2 would carry rounding of order l eps ||A||^2,
+// as large as the whole budget of a tolerance near machine epsilon.
 __global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __restrict__ s,
-                                       const double2* __restrict__ frob2, int method, double tol,
+                                       const double2* __restrict__ frob2,
+                                       const double2* __restrict__ resid2, int method, double tol,
                                        int64_t max_bond, int64_t* D_out, double* err_out,
                                        int* fallback_out) {
 #pragma clang fp contract(off)
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double cap = 0.0;
-  for (int64_t i = l - 1; i >= 0; --i) cap += s[i] * s[i];
   const double total = frob2->x;
-  const double tail = (total > cap) ? total - cap : 0.0;
+  const double tail = resid2->x > 0.0 ? resid2->x : 0.0;
   int64_t D = l;
   int fallback = 0;
   if (method == SEEMPS_TRUNC_NONE) {
@@ -123,7 +125,8 @@ constexpr int QRNB = 16;
 
 // Workspace layout of the sketch path (the fallback reuses the whole buffer).
 struct SketchWork {
-  double2 *Om, *Ycm, *Vall, *Tall, *tau, *Z, *ks, *Qcm, *Zcm, *B, *Ub, *VHb, *U, *red, *frob2;
+  double2 *Om, *Ycm, *Vall, *Tall, *tau, *Z, *ks, *Qcm, *Zcm, *B, *Ub, *VHb, *U, *red, *frob2,
+      *Rres, *resid2;
   double* S;
   int64_t* Ddev;
   double* edev;
@@ -158,6 +161,8 @@ SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
   w.U = (double2*)take(m * l * 16);
   w.red = (double2*)take(reduce_blocks(m * n) * 16);
   w.frob2 = (double2*)take(16);
+  w.Rres = (double2*)take(m * n * 16);
+  w.resid2 = (double2*)take(16);
   w.S = (double*)take(l * 8);
   w.Ddev = (int64_t*)take(8);
   w.edev = (double*)take(8);
@@ -169,8 +174,10 @@ SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
 }
 
 // Orthonormal basis of the l columns (length len) of Xcm, written to Qout
-// (column-major, l columns of length len).  Xcm is overwritten.
-int orthonormalize(double2* Xcm, int64_t len, int64_t l, double2* Qout, const SketchWork& w,
+// (column-major, l columns of length len).  Xcm is overwritten.  WS supplies the
+// QR scratch (Vall, Tall, tau, Z, ks) sized for l columns of length len.
+template <class WS>
+int orthonormalize(double2* Xcm, int64_t len, int64_t l, double2* Qout, const WS& w,
                    hipStream_t stream) {
   int st;
   if ((st = qr_factor(Xcm, len, l, QRNB, w.Vall, w.Tall, w.tau, w.Z, w.ks, 8 * QRNB * l, stream)))
@@ -178,6 +185,110 @@ int orthonormalize(double2* Xcm, int64_t len, int64_t l, double2* Qout, const Sk
   hipLaunchKernelGGL(identity_cm, dim3(grid_for(len * l)), dim3(256), 0, stream, len, l, Qout);
   if ((st = check_launch("identity_cm"))) return st;
   return qr_apply_q(Qout, len, l, l, QRNB, w.Vall, w.Tall, w.Z, w.ks, 8 * QRNB * l, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Split sketched by the bond's existing basis.  In a canonical MPS the
+// neighbour of the centre on the side the sweep moves to is an isometry whose
+// span is the previous row (or column) space of the two-site matrix; after a
+// local update that space moves only slightly, so Omega = [basis | p expansion
+// directions] captures the matrix, and W = op(A) Omega has nearly orthogonal
+// columns (an isometry times the previous singular values, whatever the gauge
+// of the other side): the Jacobi SVD of W needs no QR preconditioner and
+// converges in a few sweeps.  The expansion directions (one power step on a
+// Gaussian block, projected off the basis twice and orthonormalised) give the
+// truncation candidates outside the old space, as the two-site update needs.
+
+// Oh (l x S, row-major) = Omega^H: rows j < k from the basis (side 0: the
+// k x S row-orthonormal basis itself; side 1: the adjoint of the S x k
+// column-orthonormal basis), rows k + q = conj(column q of Qz), Qz column-major.
+__global__ void build_omega_h(int64_t k, int64_t p, int64_t S, int side,
+                              const double2* __restrict__ basis, const double2* __restrict__ Qz,
+                              double2* __restrict__ Oh) {
+  const int64_t total = (k + p) * S;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / S, c = e % S;
+    double2 v;
+    if (j < k) {
+      if (side == 0) {
+        v = basis[j * S + c];
+      } else {
+        const double2 b = basis[c * k + j];
+        v = make_double2(b.x, -b.y);
+      }
+    } else {
+      const double2 q = Qz[(j - k) * S + c];
+      v = make_double2(q.x, -q.y);
+    }
+    Oh[e] = v;
+  }
+}
+
+// dst[i][j] = conj(src[j * ld + i]) * (scale ? scale[i] : 1), i < rows, j < cols.
+__global__ void adjoint_scale(int64_t rows, int64_t cols, const double2* __restrict__ src,
+                              int64_t ld, const double* __restrict__ scale,
+                              double2* __restrict__ dst) {
+  const int64_t total = rows * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / cols, j = e % cols;
+    const double f = scale ? scale[i] : 1.0;
+    const double2 v = src[j * ld + i];
+    dst[e] = make_double2(f * v.x, -f * v.y);
+  }
+}
+
+// Workspace of the basis-sketched split: Lg x Sh working orientation (Lg = m,
+// Sh = n for side 0; Lg = n, Sh = m for side 1), l = k + p sketch columns.
+struct BasisWork {
+  double2 *G0t, *Yt, *Zt, *C, *Qz, *Vall, *Tall, *tau, *Z, *ks, *Oh, *W, *Uw, *VHw, *T, *red,
+      *frob2, *Rres, *resid2;
+  double* S;
+  int64_t* Ddev;
+  double* edev;
+  int* fdev;
+  void* svdw;
+  int64_t svd_bytes;
+  int64_t bytes;
+};
+
+BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) {
+  const int64_t l = k + p, mx = Lg > Sh ? Lg : Sh, pp = p > 0 ? p : 1;
+  BasisWork w;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    void* q = base ? (void*)((char*)base + off) : nullptr;
+    off += al256(bytes);
+    return q;
+  };
+  w.G0t = (double2*)take(pp * Sh * 16);
+  w.Yt = (double2*)take(pp * Lg * 16);
+  w.Zt = (double2*)take(pp * Sh * 16);
+  w.C = (double2*)take(pp * k * 16);
+  w.Qz = (double2*)take(pp * Sh * 16);
+  w.Vall = (double2*)take(pp * Sh * 16);
+  w.Tall = (double2*)take(ceil_div(pp, QRNB) * QRNB * QRNB * 16);
+  w.tau = (double2*)take(pp * 16);
+  w.Z = (double2*)take(2 * QRNB * (Sh > pp ? Sh : pp) * 16);
+  w.ks = (double2*)take(8 * QRNB * pp * 16);
+  w.Oh = (double2*)take(l * Sh * 16);
+  w.W = (double2*)take(Lg * l * 16);
+  w.Uw = (double2*)take(Lg * l * 16);
+  w.VHw = (double2*)take(l * l * 16);
+  w.T = (double2*)take(l * mx * 16);
+  w.red = (double2*)take(reduce_blocks(Lg * Sh) * 16);
+  w.frob2 = (double2*)take(16);
+  w.Rres = (double2*)take(Lg * Sh * 16);
+  w.resid2 = (double2*)take(16);
+  w.S = (double*)take(l * 8);
+  w.Ddev = (int64_t*)take(8);
+  w.edev = (double*)take(8);
+  w.fdev = (int*)take(8);
+  w.svd_bytes = svd_workspace_bytes(Lg, l, false);
+  w.svdw = take(w.svd_bytes);
+  w.bytes = off;
+  return w;
 }
 
 constexpr int NF = 32;  // sketch columns = features of the rank predictor
@@ -337,8 +448,14 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
       return st;
   }
   if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
+  // R = A - Q B (m x n), Q^T stored l x m
+  SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
+  if ((st = zgemm(SEEMPS_OP_T, SEEMPS_OP_N, m, n, l, make_double2(-1.0, 0.0), w.Qcm, m, 0, w.B, n,
+                  0, one, w.Rres, n, 0, 1, st_)))
+    return st;
+  if ((st = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return st;
   hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
-                     method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
+                     w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
   if ((st = check_launch("truncate_sketch"))) return st;
   if ((st = svd_complete_enqueue(l, n, w.Ub, w.VHb, w.svdw, w.Ddev, st_))) return st;
   int64_t hD = 0;
@@ -360,6 +477,169 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
   } else {
     if ((st = zscale2d_impl(m, D, nullptr, w.S, w.U, l, (double2*)outA, D, st_))) return st;
     if ((st = zscale2d_impl(D, n, nullptr, nullptr, w.VHb, n, (double2*)outB, n, st_))) return st;
+  }
+  if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, w.S, l * 8, hipMemcpyDeviceToDevice, st_));
+  *D_host = D;
+  *err_host = herr;
+  return SEEMPS_OK;
+}
+
+extern "C" int64_t seemps_svd_split_basis_workspace(int64_t m, int64_t n, int side, int64_t k,
+                                                    int64_t p) {
+  const int64_t full = seemps::svd_split_bytes(m, n);
+  const int64_t Lg = side == 0 ? m : n, Sh = side == 0 ? n : m;
+  if (k < 1 || p < 0 || k + p > (Lg < Sh ? Lg : Sh)) return full;
+  const int64_t bk = seemps::basis_carve(Lg, Sh, k, p, nullptr).bytes;
+  return bk > full ? bk : full;
+}
+
+extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int side,
+                                      const void* basis, int64_t k, int64_t p, uint64_t seed,
+                                      int mode, int method, double tolerance, int64_t max_bond,
+                                      void* outA, void* outB, double* s_out, int64_t* D_host,
+                                      double* err_host, int32_t* fallback_host, void* work,
+                                      int64_t work_bytes, int32_t* info, void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(m >= 1 && n >= 1, "svd_split_basis: empty matrix");
+  SEEMPS_CHECK_ARG(mode == 0 || mode == 1, "svd_split_basis: invalid mode");
+  SEEMPS_CHECK_ARG(side == 0 || side == 1, "svd_split_basis: invalid side");
+  SEEMPS_CHECK_ARG(k >= 1 && p >= 0, "svd_split_basis: invalid sketch sizes");
+  SEEMPS_CHECK_ARG(work_bytes >= seemps_svd_split_basis_workspace(m, n, side, k, p),
+                   "svd_split_basis: workspace too small");
+  hipStream_t st_ = (hipStream_t)stream;
+  const double2* A = (const double2*)A_;
+  const double2* Bs = (const double2*)basis;
+  const int64_t r = m < n ? m : n, l = k + p;
+  const int64_t Lg = side == 0 ? m : n, Sh = side == 0 ? n : m;
+  if (fallback_host) *fallback_host = 0;
+  auto full_split = [&]() {
+    if (fallback_host) *fallback_host = 1;
+    return svd_split_impl(m, n, A, mode, method, tolerance, max_bond, (double2*)outA,
+                          (double2*)outB, s_out, D_host, err_host, work, work_bytes, info, st_);
+  };
+  if (l > (Lg < Sh ? Lg : Sh) || method == SEEMPS_TRUNC_NONE) return full_split();
+  const BasisWork w = basis_carve(Lg, Sh, k, p, work);
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0),
+                mone = make_double2(-1.0, 0.0);
+  int st;
+  {
+    TimedLaunch phase("phase_basis_sketch", st_, 0.0);
+    if (p > 0) {
+      // Expansion block Z = X^H X G0 (Sh x p), X = A (side 0) or A^H (side 1),
+      // kept transposed: Zt = Z^T (p x Sh), which is Z column-major.
+      hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(p * Sh)), dim3(256), 0, st_, p * Sh, seed,
+                         w.G0t);
+      if ((st = check_launch("gauss"))) return st;
+      if (side == 0) {
+        // Yt = G0^T A^T (p x m); Zt = Yt conj(A) (p x n)
+        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.G0t, n, 0, A, n, 0, zero, w.Yt,
+                        m, 0, 1, st_)))
+          return st;
+        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.Yt, m, 0, A, n, 0, zero, w.Zt,
+                        n, 0, 1, st_)))
+          return st;
+      } else {
+        // Yt = G0^T conj(A) (p x n); Zt = Yt A^T (p x m)
+        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.G0t, m, 0, A, n, 0, zero, w.Yt,
+                        n, 0, 1, st_)))
+          return st;
+        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.Yt, n, 0, A, n, 0, zero, w.Zt,
+                        m, 0, 1, st_)))
+          return st;
+      }
+      // X -= Q (Q^H X) for the p columns of Xt (column-major, length Sh);
+      // transposed: C = Xt conj(Q) (p x k), Xt -= C Q^T.
+      auto project = [&](double2* Xt) -> int {
+        int e;
+        if (side == 0) {  // Q^H = basis (k x n): conj(Q) = basis^T, Q^T = conj(basis)
+          if ((e = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, k, n, one, Xt, n, 0, Bs, n, 0, zero, w.C, k,
+                         0, 1, st_)))
+            return e;
+          return zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, k, mone, w.C, k, 0, Bs, n, 0, one, Xt, n,
+                       0, 1, st_);
+        }
+        // Q = basis (m x k): conj(Q) = conj(basis), Q^T = basis^T
+        if ((e = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, k, m, one, Xt, m, 0, Bs, k, 0, zero, w.C, k,
+                       0, 1, st_)))
+          return e;
+        return zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, k, mone, w.C, k, 0, Bs, k, 0, one, Xt, m, 0,
+                     1, st_);
+      };
+      // Project twice, orthonormalise; then once more of each: when A lies (almost)
+      // in the basis span, Z is rank-deficient and the Householder factor's extra
+      // columns are arbitrary, so they must be projected off the basis again.
+      if ((st = project(w.Zt)) || (st = project(w.Zt))) return st;
+      if ((st = orthonormalize(w.Zt, Sh, p, w.Qz, w, st_))) return st;
+      if ((st = project(w.Qz)) || (st = project(w.Qz))) return st;
+      if ((st = orthonormalize(w.Qz, Sh, p, w.Zt, w, st_))) return st;
+    }
+    const double2* Qext = w.Zt;  // p orthonormal columns orthogonal to the basis
+    hipLaunchKernelGGL(build_omega_h, dim3(grid_for(l * Sh)), dim3(256), 0, st_, k, p, Sh, side,
+                       Bs, Qext, w.Oh);
+    if ((st = check_launch("build_omega_h"))) return st;
+    // W = X Omega = X Oh^H (Lg x l)
+    if (side == 0) {
+      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_H, m, l, n, one, A, n, 0, w.Oh, n, 0, zero, w.W, l,
+                      0, 1, st_)))
+        return st;
+    } else {
+      if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, n, l, m, one, A, n, 0, w.Oh, m, 0, zero, w.W, l,
+                      0, 1, st_)))
+        return st;
+    }
+  }
+  int off = 0;
+  {
+    TimedLaunch phase("phase_basis_svd", st_, 0.0);
+    if ((st = svd_enqueue(Lg, l, w.W, l, w.Uw, w.S, w.VHw, w.svdw, w.svd_bytes, st_, &off, false,
+                          false)))
+      return st;
+  }
+  if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
+  // R = A - (projection of A on Omega): side 0: A - W Oh; side 1: A - Oh^H W^H
+  SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
+  if (side == 0) {
+    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, m, n, l, mone, w.W, l, 0, w.Oh, n, 0, one, w.Rres,
+                    n, 0, 1, st_)))
+      return st;
+  } else {
+    if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, m, n, l, mone, w.Oh, m, 0, w.W, l, 0, one, w.Rres,
+                    n, 0, 1, st_)))
+      return st;
+  }
+  if ((st = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return st;
+  hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
+                     w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
+  if ((st = check_launch("truncate_sketch"))) return st;
+  if ((st = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return st;
+  int64_t hD = 0;
+  double herr = 0.0;
+  int hfb = 0;
+  SEEMPS_HIP(hipMemcpyAsync(&hD, w.Ddev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(&herr, w.edev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(&hfb, w.fdev, sizeof(int), hipMemcpyDeviceToHost, st_));
+  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;
+  if (hfb) return full_split();
+  const int64_t D = hD;
+  const double* sA = (mode == 1) ? w.S : nullptr;  // S into the left factor
+  const double* sB = (mode == 0) ? w.S : nullptr;  // S into the right factor
+  if (side == 0) {
+    // A = Uw S (VHw Oh): left = Uw[:, :D], right = VHw[:D] Oh (D x n)
+    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, D, n, l, one, w.VHw, l, 0, w.Oh, n, 0, zero, w.T, n,
+                    0, 1, st_)))
+      return st;
+    if ((st = zscale2d_impl(m, D, nullptr, sA, w.Uw, l, (double2*)outA, D, st_))) return st;
+    if ((st = zscale2d_impl(D, n, sB, nullptr, w.T, n, (double2*)outB, n, st_))) return st;
+  } else {
+    // A^H = Uw S (VHw Oh): A = (Oh^H VHw^H) S Uw^H; left = Oh^H VHw[:D]^H (m x D),
+    // right = Uw[:, :D]^H (D x n)
+    if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, m, D, l, one, w.Oh, m, 0, w.VHw, l, 0, zero, w.T, D,
+                    0, 1, st_)))
+      return st;
+    if ((st = zscale2d_impl(m, D, nullptr, sA, w.T, D, (double2*)outA, D, st_))) return st;
+    hipLaunchKernelGGL(adjoint_scale, dim3(grid_for(D * n)), dim3(256), 0, st_, D, n, w.Uw, l, sB,
+                       (double2*)outB);
+    if ((st = check_launch("adjoint_scale"))) return st;
   }
   if (s_out) SEEMPS_HIP(hipMemcpyAsync(s_out, w.S, l * 8, hipMemcpyDeviceToDevice, st_));
   *D_host = D;

@@ -90,14 +90,14 @@ struct SvdWork {
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
-SvdShape svd_shape(int64_t m, int64_t n, int num_cu) {
+SvdShape svd_shape(int64_t m, int64_t n, int num_cu, bool precondition = true) {
   SvdShape s;
   s.m = m;
   s.n = n;
   s.transposed = (m < n);
   s.lenA = s.transposed ? n : m;
   s.ncol = s.transposed ? m : n;
-  s.use_qr = (s.ncol >= QR_MIN_COLS);
+  s.use_qr = precondition && (s.ncol >= QR_MIN_COLS);
   s.len = s.use_qr ? s.ncol : s.lenA;
   int64_t nb = ceil_div(s.ncol, BW);
   if (nb < 2) nb = 2;
@@ -952,11 +952,11 @@ unsigned blocks_for(int64_t n) {
 // svd_complete_enqueue must follow, limited to the slots the caller keeps.
 int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U, double* S,
                 double2* VH, void* work, int64_t work_bytes, hipStream_t stream,
-                int* info_dev_off, bool complete) {
+                int* info_dev_off, bool complete, bool precondition) {
   DeviceInfo di;
   int st = device_info(&di);
   if (st) return st;
-  const SvdShape s = svd_shape(m, n, di.num_cu);
+  const SvdShape s = svd_shape(m, n, di.num_cu, precondition);
   SEEMPS_CHECK_ARG(work_bytes >= svd_bytes(s), "svd: workspace too small (%lld < %lld)",
                    (long long)work_bytes, (long long)svd_bytes(s));
   SvdWork w = svd_carve(s, work);
@@ -1029,20 +1029,20 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
 
 // Orthonormal completion of the slots k < *kmax_dev (see svd_complete).
 int svd_complete_enqueue(int64_t m, int64_t n, double2* U, double2* VH, void* work,
-                         const int64_t* kmax_dev, hipStream_t stream) {
+                         const int64_t* kmax_dev, hipStream_t stream, bool precondition) {
   DeviceInfo di;
   int st = device_info(&di);
   if (st) return st;
-  const SvdShape s = svd_shape(m, n, di.num_cu);
+  const SvdShape s = svd_shape(m, n, di.num_cu, precondition);
   SvdWork w = svd_carve(s, work);
   TimedLaunch timed("svd_complete", stream, 0.0);
   hipLaunchKernelGGL(svd_complete, dim3(1), dim3(1024), 0, stream, s, w, U, VH, kmax_dev);
   return check_launch("svd_complete");
 }
 
-int64_t svd_workspace_bytes(int64_t m, int64_t n) {
+int64_t svd_workspace_bytes(int64_t m, int64_t n, bool precondition) {
   // Sized for the largest row split so the size does not depend on the device.
-  SvdShape s = svd_shape(m, n, 1 << 20);
+  SvdShape s = svd_shape(m, n, 1 << 20, precondition);
   return svd_bytes(s);
 }
 

@@ -212,6 +212,35 @@ def svd_split_sketch(AA: torch.Tensor, m: int, n: int, mode: int, method: int, t
             None if s is None else s[:Dv], bool(fb.value))
 
 
+def svd_split_basis(AA: torch.Tensor, m: int, n: int, side: int, basis: torch.Tensor, k: int,
+                    extra: int, mode: int, method: int, tol: float, max_bond: int, seed: int = 0,
+                    want_s: bool = False):
+    """Split of the m x n matrix AA sketched by an isometry of the bond.
+
+    side 0: `basis` is k x n with orthonormal rows (the right neighbour of a
+    centre moving right); side 1: m x k with orthonormal columns (the left
+    neighbour of a centre moving left).  `extra` expansion directions are
+    added to the sketch.  Returns (A, B, D, err^2, s or None, fallback) like
+    :func:`svd_split_sketch`."""
+    r = min(m, n)
+    basis = as_dev(basis)
+    outA = empty(m * r)
+    outB = empty(r * n)
+    s = torch.empty(r, dtype=torch.float64, device=AA.device) if want_s else None
+    D = ctypes.c_int64(0)
+    err = ctypes.c_double(0.0)
+    fb = ctypes.c_int32(0)
+    info = (ctypes.c_int32 * 2)()
+    wp, wn = workspace.get(lib.seemps_svd_split_basis_workspace(m, n, side, k, extra))
+    check(lib.seemps_svd_split_basis(m, n, ptr(AA), side, ptr(basis), k, extra,
+                                     seed & 0xFFFFFFFFFFFFFFFF, mode, method, tol, max_bond,
+                                     ptr(outA), ptr(outB), ptr(s), ctypes.byref(D),
+                                     ctypes.byref(err), ctypes.byref(fb), wp, wn, info, stream()))
+    Dv = D.value
+    return (outA[: m * Dv].view(m, Dv), outB[: Dv * n].view(Dv, n), Dv, err.value,
+            None if s is None else s[:Dv], bool(fb.value))
+
+
 def permute(x: torch.Tensor, perm) -> torch.Tensor:
     dims = (ctypes.c_int64 * x.dim())(*x.shape)
     p = (ctypes.c_int * x.dim())(*perm)

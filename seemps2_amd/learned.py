@@ -11,6 +11,13 @@ too small.  Training data are the spectra of the plain path's own full SVDs
 (:class:`SpectrumLog`), and training runs on the device through the HIP
 forward, backward and Adam kernels (``seemps_mlp_*``).
 
+When the split comes with the bond's current basis (the canonical neighbour
+that a DMRG sweep is about to replace), the sketch is that basis plus
+p = max(oversample, l - k) expansion directions (``seemps_svd_split_basis``):
+the old basis already spans almost all of the new two-site matrix, so the
+Jacobi SVD of the sketched matrix needs neither a QR preconditioner nor many
+sweeps, and the predicted rank decides how far the bond may grow beyond it.
+
 Nothing here changes the reference-compatible default path: the learned
 splits are used only inside ``with learned_truncation(...)``.
 """
@@ -155,9 +162,11 @@ class LearnedTruncation:
     power_iters: int = 2
     min_size: int = 64
     seed: int = 1234
+    use_basis: bool = True
     calls: int = 0
     fallbacks: int = 0
     full_splits: int = 0
+    basis_splits: int = 0
     sketch_columns: int = 0
 
     def sketch_width(self, feats: torch.Tensor, m: int, n: int, max_bond: int) -> int:
@@ -167,8 +176,12 @@ class LearnedTruncation:
         l = int(math.ceil(self.margin * frac * r)) + self.oversample
         return min(l, max_bond + self.oversample, r)
 
-    def split(self, AA2: torch.Tensor, m: int, n: int, mode: int, strategy):
-        """(A, B, D, err^2) of the two-site split, by sketch when it pays."""
+    def split(self, AA2: torch.Tensor, m: int, n: int, mode: int, strategy, basis=None):
+        """(A, B, D, err^2) of the two-site split, by sketch when it pays.
+
+        ``basis``: the canonical neighbour the split replaces (mode 0: the
+        right site, k x d x b, right-canonical; mode 1: the left site,
+        a x d x k, left-canonical), or None."""
         r = min(m, n)
         self.calls += 1
         if r < self.min_size:
@@ -177,6 +190,18 @@ class LearnedTruncation:
                                strategy.max_bond_dimension)[:4]
         feats = spectral_features(AA2, m, n, seed=self.seed + self.calls)
         l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
+        if basis is not None and self.use_basis:
+            k = basis.shape[0] if mode == 0 else basis.shape[-1]
+            p = max(self.oversample, l - k)
+            if k + p < r:
+                bm = basis.reshape(k, n) if mode == 0 else basis.reshape(m, k)
+                A, B, D, err2, _, fb = K.svd_split_basis(
+                    AA2, m, n, mode, bm, k, p, mode, strategy.method, strategy.tolerance,
+                    strategy.max_bond_dimension, seed=self.seed + 7919 * self.calls)
+                self.basis_splits += 1
+                self.sketch_columns += k + p
+                self.fallbacks += int(fb)
+                return A, B, D, err2
         if l >= r:
             self.full_splits += 1
             return K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
@@ -217,10 +242,12 @@ def recording(log: SpectrumLog):
         _RECORDING = prev
 
 
-def two_site_split(AA2: torch.Tensor, m: int, n: int, mode: int, strategy):
-    """The split used by the two-site updates: plain, learned or recorded."""
+def two_site_split(AA2: torch.Tensor, m: int, n: int, mode: int, strategy, basis=None):
+    """The split used by the two-site updates: plain, learned or recorded.
+    ``basis`` is the optional canonical neighbour (see LearnedTruncation.split);
+    the plain split ignores it."""
     if _ACTIVE is not None:
-        return _ACTIVE.split(AA2, m, n, mode, strategy)
+        return _ACTIVE.split(AA2, m, n, mode, strategy, basis)
     A, B, D, err2, _ = K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
                                    strategy.max_bond_dimension)
     if _RECORDING is not None and min(m, n) >= _RECORDING.min_size:

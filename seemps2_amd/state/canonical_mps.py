@@ -132,13 +132,17 @@ class CanonicalMPS(MPS):
 
     def update_2site_right(self, AA, site: int, strategy: Strategy) -> None:
         """Left-orthogonal split of AA into sites (site, site+1) (canonical_mps.py:326-345)."""
-        self._data[site], self._data[site + 1], error = _left_orth_2site(AA, strategy)
+        # with the centre at `site`, site + 1 is right-canonical: a sketch basis
+        basis = self._data[site + 1] if self._center == site else None
+        self._data[site], self._data[site + 1], error = _left_orth_2site(AA, strategy, basis)
         self._center = site + 1
         self._error += error
 
     def update_2site_left(self, AA, site: int, strategy: Strategy) -> None:
         """Right-orthogonal split of AA into sites (site, site+1) (canonical_mps.py:347-366)."""
-        self._data[site], self._data[site + 1], error = _right_orth_2site(AA, strategy)
+        # with the centre at `site + 1`, site is left-canonical: a sketch basis
+        basis = self._data[site] if self._center == site + 1 else None
+        self._data[site], self._data[site + 1], error = _right_orth_2site(AA, strategy, basis)
         self._center = site
         self._error += error
 

@@ -353,3 +353,102 @@ def test_svd_split_sketch_matches_full(K, rng, m, n, rate, tol, mb):
     assert not fb and D == cap == Ar.shape[-1]
     assert rel(K.to_numpy(Ad) @ K.to_numpy(Bd), Ar.reshape(m, -1) @ Br.reshape(cap, -1)) < \
         _sketch_bound(sv, cap, lc, 2)
+
+
+def _in_span(rng, m, n, k, side, rate=0.05):
+    """A (m x n) whose row (side 0) or column (side 1) space is spanned by a
+    k-dimensional isometry, and that isometry in the layout the split takes."""
+    g = np.exp(-rate * np.arange(k))
+    if side == 0:
+        Q, _ = np.linalg.qr(crandn(rng, n, k))
+        basis = Q.conj().T                       # k x n, orthonormal rows
+        A = (crandn(rng, m, k) * g) @ basis
+    else:
+        basis, _ = np.linalg.qr(crandn(rng, m, k))  # m x k, orthonormal columns
+        A = basis @ (crandn(rng, k, n) * g[:, None])
+    return A, basis
+
+
+def _outside(rng, basis, side, m, n, r, scale):
+    """Rank-r term whose row (side 0) / column (side 1) space is orthogonal to the basis."""
+    if side == 0:
+        X = crandn(rng, n, r)
+        X -= basis.conj().T @ (basis @ X)
+        return scale * crandn(rng, m, r) @ np.linalg.qr(X)[0].conj().T
+    X = crandn(rng, m, r)
+    X -= basis @ (basis.conj().T @ X)
+    return scale * np.linalg.qr(X)[0] @ crandn(rng, r, n)
+
+
+def _check_split(K, A, Ad, Bd, D, mode, Ar, Br, tol_rec):
+    assert D == Ar.shape[-1]
+    a, b = K.to_numpy(Ad), K.to_numpy(Bd)
+    iso = a.conj().T @ a if mode == 0 else b @ b.conj().T
+    assert np.abs(iso - np.eye(D)).max() < 1e-12
+    ref = Ar.reshape(A.shape[0], -1) @ Br.reshape(D, -1)
+    assert rel(a @ b, ref) < tol_rec
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_svd_split_basis_exact_span(K, rng, side, mode):
+    """A inside the basis span: the basis-sketched split equals the full split
+    (same D, reconstruction and isometry to rounding), with no fallback, also
+    though the expansion block is then pure rounding noise."""
+    m, n, k = 192, 160, 96
+    A, basis = _in_span(rng, m, n, k, side)
+    st = ocore.Strategy(tolerance=1e-12, max_bond_dimension=1000)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), st)
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(A), m, n, side, K.as_dev(basis), k, 16,
+                                               mode, st.method, st.tolerance,
+                                               st.max_bond_dimension, seed=5)
+    assert not fb
+    _check_split(K, A, Ad, Bd, D, mode, Ar, Br, 1e-12)
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_svd_split_basis_grows_and_caps(K, rng, side):
+    """A rank-4 term outside the basis: the expansion directions let the bond
+    grow to k + 4 exactly as the full split does; with the bond capped at k the
+    sketch decides alone and keeps the best k directions within the 1e-9 bound;
+    a generic perturbation above the tolerance falls back to the full split."""
+    m, n, k = 256, 224, 100
+    A0, basis = _in_span(rng, m, n, k, side, rate=0.02)
+    A = A0 + _outside(rng, basis, side, m, n, 4, 0.3)
+    mode = side
+    st = ocore.Strategy(tolerance=1e-12, max_bond_dimension=1000)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), st)
+    assert Ar.shape[-1] == k + 4
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(A), m, n, side, K.as_dev(basis), k, 16,
+                                               mode, st.method, st.tolerance,
+                                               st.max_bond_dimension, seed=9)
+    assert not fb
+    _check_split(K, A, Ad, Bd, D, mode, Ar, Br, 1e-11)
+    # capped at k: the sketch (k + 16 columns) holds the whole matrix, so the
+    # capped split is the full one's
+    stc = ocore.Strategy(tolerance=1e-12, max_bond_dimension=k)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), stc)
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(A), m, n, side, K.as_dev(basis), k, 16,
+                                               mode, stc.method, stc.tolerance, k, seed=9)
+    assert not fb and D == k
+    _check_split(K, A, Ad, Bd, D, mode, Ar, Br, 1e-11)
+    assert abs(np.sqrt(err2) - er) < 1e-9 * np.linalg.norm(A)
+    # generic noise outside the span, far above the tolerance: only the full split decides
+    An = A0 + 1e-4 * crandn(rng, m, n)
+    Ar, Br, er = ocore.left_orth_2site(An.reshape(m, 1, 1, n), st)
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(An), m, n, side, K.as_dev(basis), k, 16,
+                                               mode, st.method, st.tolerance,
+                                               st.max_bond_dimension, seed=9)
+    assert fb
+    _check_split(K, An, Ad, Bd, D, mode, Ar, Br, 1e-12)
+
+
+def test_svd_split_basis_too_wide_falls_back(K, rng):
+    m, n, k = 64, 48, 40
+    A, basis = _in_span(rng, m, n, k, 0)
+    st = ocore.Strategy(tolerance=1e-12, max_bond_dimension=1000)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), st)
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(A), m, n, 0, K.as_dev(basis), k, 16,
+                                               0, st.method, st.tolerance, 1000)
+    assert fb  # k + p > min(m, n)
+    _check_split(K, A, Ad, Bd, D, 0, Ar, Br, 1e-12)

@@ -140,3 +140,10 @@ def test_learned_dmrg_energy_matches_plain(L):
         learned = S.dmrg(H, guess, strategy=st, maxiter=6)
     assert abs(learned.energy - plain.energy) <= 1e-9 * abs(plain.energy)
     assert lt.calls > 0
+    assert lt.basis_splits > 0  # the canonical neighbours served as sketches
+    # the same run with random sketches only
+    lt2 = L.LearnedTruncation(net, min_size=8, oversample=8, use_basis=False)
+    with L.learned_truncation(lt2):
+        learned2 = S.dmrg(H, guess, strategy=st, maxiter=6)
+    assert abs(learned2.energy - plain.energy) <= 1e-9 * abs(plain.energy)
+    assert lt2.basis_splits == 0
