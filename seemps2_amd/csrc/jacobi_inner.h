@@ -85,9 +85,12 @@ __device__ __forceinline__ void inner_pair(int k, int t, int* a, int* b) {
 // of one per wavefront); then thread (p, q) applies J_p^H G[pair p, pair q]
 // J_q to one 2x2 block, reading G and writing the other buffer, and updates
 // two rows of Q for pair p; a round whose 16 rotations are all the identity
-// skips that phase.  Returns the buffer holding the final G.
+// skips that phase.  Returns the buffer holding the final G.  When nactive is
+// given, *nactive (thread-private, the same value in every thread) receives
+// the number of rounds that rotated anything.
 __device__ inline double2* inner_jacobi(double2* G, double2* G2, double2* Q, double2* rot, int* active,
-                                 double tol, double tiny2, int sweeps) {
+                                 double tol, double tiny2, int sweeps,
+                                 int* nactive = nullptr) {
   const int tid = threadIdx.x;
   const int p = tid >> 4, q = tid & 15;
   double2* cur = G;
@@ -114,6 +117,7 @@ __device__ inline double2* inner_jacobi(double2* G, double2* G2, double2* Q, dou
       // identity round: G and Q unchanged, buffers not swapped.  The flag is
       // double-buffered so a wavefront still reading it is never overtaken.
       if (!active[(rnd++) & 1]) continue;
+      if (nactive) ++*nactive;
       int a, b, c, d;
       inner_pair(k, p, &a, &b);
       inner_pair(k, q, &c, &d);

@@ -393,9 +393,10 @@ __global__ __launch_bounds__(64) void lz_tridiag(LzScalars* s, int m) {
   }
   if (lane != 0) return;
   const double theta = 0.5 * (lo_s + hi_s);
-  // Inverse iteration on (T - theta I).
-  double d[64], dl[64], du[64], du2[64], y[64];
-  int piv[64];
+  // Inverse iteration on (T - theta I), one lane; the factors live in LDS
+  // (dynamically indexed private arrays would spill to scratch memory).
+  __shared__ double d[64], dl[64], du[64], du2[64], y[64];
+  __shared__ int piv[64];
   double tnorm = 0.0;
   for (int i = 0; i < kd; ++i) tnorm = fmax(tnorm, fabs(a[i]) + (i < kd - 1 ? fabs(b[i]) : 0.0));
   const double tiny = 2.220446049250313e-16 * fmax(tnorm, 1e-300);

@@ -316,8 +316,10 @@ __global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __r
   __syncthreads();
   double2* cur = A;
   for (int sw = 0; sw < 12; ++sw) {
-    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-15, 0.0, 1);
+    int nact = 0;
+    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-12, 0.0, 1, &nact);
     __syncthreads();  // the next call rewrites the round flags
+    if (nact == 0) break;  // a sweep without rotations: converged (uniform)
   }
   if (tid < PW) lam[tid] = cur[tid * GP + tid].x;
   __syncthreads();
