@@ -92,6 +92,13 @@ int seemps_zgemm_prec(int precision, int opA, int opB, int64_t M, int64_t N, int
                       void* C, int64_t ldc, int64_t strideC,
                       int64_t batch, void* stream);
 
+/* Site-wise product of two MPS tensors (the MPS Hadamard product, one site):
+ * out[(x, y), k, (u, v)] = A[x, k, u] B[y, k, v], A (a, d, b), B (c, d, e),
+ * out (a*c, d, b*e).  Replaces the broadcast product of MPS.__mul__(MPS)
+ * (state/mps.py:201-210). */
+int seemps_zsite_product(int64_t a, int64_t d, int64_t b, int64_t c, int64_t e,
+                         const void* A, const void* B, void* out, void* stream);
+
 /* Local-operator contraction, complex128:
  *   out[p, j1, q, j2] = sum_i in[p, i, q] * Mop[i, j1, j2]   (+ out if accumulate)
  * with in[P, I, Q], Mop[I, J1, J2], out[P, J1, Q, J2] contiguous, 64*I + I*J1*J2 <= 3072.  Replaces the small-index steps of _mpo_multiply_tensor

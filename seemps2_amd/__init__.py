@@ -34,7 +34,17 @@ from .state import (
     random_mps,
     random_uniform_mps,
 )
-from .operators import MPO, QuadraticForm
+from .operators import MPO, MPOSum, QuadraticForm
+from .state import mps_tensor_product
+from .analysis import (
+    QuantizedInterval,
+    RegularInterval,
+    finite_differences_mpo,
+    interleaving_permutation,
+    tridiagonal_mpo,
+)
+from .register import mpo_shifts, mpo_weighted_shifts
+from .evolution import euler
 from .hamiltonians import (
     ConstantNNHamiltonian,
     ConstantTIHamiltonian,
@@ -49,6 +59,16 @@ cython = core
 _sys.modules[__name__ + ".cython"] = core
 
 __all__ = [
+    "MPOSum",
+    "QuantizedInterval",
+    "RegularInterval",
+    "euler",
+    "finite_differences_mpo",
+    "interleaving_permutation",
+    "mpo_shifts",
+    "mpo_weighted_shifts",
+    "mps_tensor_product",
+    "tridiagonal_mpo",
     "AntilinearForm",
     "CanonicalMPS",
     "ConstantNNHamiltonian",

@@ -58,6 +58,21 @@ __global__ void zaxpby_kernel(int64_t n, double2 a, const double2* __restrict__ 
   }
 }
 
+// Site-wise product of two MPS tensors (the MPS Hadamard product):
+// out[(x, y), k, (u, v)] = A[x, k, u] * B[y, k, v], A (a, d, b), B (c, d, e).
+__global__ void zsite_product_kernel(int64_t a, int64_t d, int64_t b, int64_t c, int64_t e,
+                                     const double2* __restrict__ A, const double2* __restrict__ B,
+                                     double2* __restrict__ out) {
+  const int64_t be = b * e, total = a * c * d * be;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t col = i % be, row = i / be;  // row = (x * c + y) * d + k
+    const int64_t k = row % d, xy = row / d;
+    const int64_t x = xy / c, y = xy % c, u = col / e, v = col % e;
+    out[i] = cmul(A[(x * d + k) * b + u], B[(y * d + k) * e + v]);
+  }
+}
+
 __global__ void zscale2d_kernel(int64_t rows, int64_t cols, const double* __restrict__ sr,
                                 const double* __restrict__ sc, const double2* __restrict__ src,
                                 int64_t ld_src, double2* __restrict__ dst, int64_t ld_dst) {
@@ -214,4 +229,18 @@ extern "C" int seemps_zdotc(int64_t n, const void* x, const void* y, void* resul
   }
   return seemps::zmdotc_impl(n, 1, (const double2*)x, n, (const double2*)y, (double2*)result,
                              (double2*)work, (hipStream_t)stream);
+}
+
+extern "C" int seemps_zsite_product(int64_t a, int64_t d, int64_t b, int64_t c, int64_t e,
+                                    const void* A, const void* B, void* out, void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(a >= 1 && d >= 1 && b >= 1 && c >= 1 && e >= 1,
+                   "site_product: bad dimensions");
+  const int64_t total = a * c * d * b * e;
+  int64_t blocks = ceil_div(total, 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(zsite_product_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, a, d, b, c, e, (const double2*)A, (const double2*)B,
+                     (double2*)out);
+  return check_launch("site_product");
 }

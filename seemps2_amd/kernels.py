@@ -129,6 +129,17 @@ def embed_block(B: torch.Tensor, A: torch.Tensor, l_off: int, r_off: int) -> Non
     check(lib.seemps_zscale2d(a * d, b, None, None, ptr(A), b, dst, DR, stream()))
 
 
+def site_product(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """out[(x,y), k, (u,v)] = A[x,k,u] B[y,k,v] for MPS tensors A (a,d,b), B (c,d,e)."""
+    a, d, b = A.shape
+    c, d2, e = B.shape
+    if d2 != d:
+        raise ValueError("site_product: physical dimensions differ")
+    out = empty(a * c, d, b * e)
+    check(lib.seemps_zsite_product(a, d, b, c, e, ptr(A), ptr(B), ptr(out), stream()))
+    return out
+
+
 def dotc_dev(x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     """Device scalar sum(conj(x) * y)."""
     n = x.numel()

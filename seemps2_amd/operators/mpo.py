@@ -181,5 +181,52 @@ class MPO(TensorArray):
     def reverse(self) -> "MPO":
         return MPO([K.permute(A, (3, 1, 2, 0)) for A in reversed(self._data)], self.strategy)
 
+    def __add__(self, A):
+        """self + A as an MPOSum (mpo.py:87-101)."""
+        from .mposum import MPOSum
+
+        if isinstance(A, MPO):
+            return MPOSum([self, A], [1.0, 1.0], self.strategy)
+        if isinstance(A, MPOSum):
+            return MPOSum([self] + A.mpos, [1.0] + A.weights, self.strategy)
+        raise TypeError(f"Cannot add MPO and {type(A)}")
+
+    def __sub__(self, A):
+        from .mposum import MPOSum
+
+        if isinstance(A, MPO):
+            return MPOSum([self, A], [1.0, -1.0], self.strategy)
+        if isinstance(A, MPOSum):
+            return MPOSum([self] + A.mpos, [1.0] + [-w for w in A.weights], self.strategy)
+        raise TypeError(f"Cannot subtract {type(A)} from MPO")
+
+    def extend(self, L: int, sites=None, dimensions=2) -> "MPO":
+        """The operator on L sites: this MPO's tensors at `sites` (default the
+        first ones), identities of the given physical dimensions elsewhere,
+        each carrying the bond of the tensor to its left (mpo.py:326-376)."""
+        if isinstance(dimensions, int):
+            extra = [dimensions] * max(L - self.size, 0)
+        else:
+            extra = list(dimensions)
+            if len(extra) != L - self.size:
+                raise ValueError("len(dimensions) must equal L - self.size")
+        if L < self.size:
+            raise ValueError("New size L must be at least the current size")
+        sites = list(range(self.size)) if sites is None else list(sites)
+        if len(sites) != self.size:
+            raise ValueError("len(sites) must equal the current size")
+        placed = dict(zip(sites, self._data))
+        data, D, k = [], 1, 0
+        for i in range(L):
+            if i in placed:
+                A = placed[i]
+                D = A.shape[-1]
+            else:
+                d = extra[k]
+                k += 1
+                A = np.einsum("ab,ij->aijb", np.eye(D), np.eye(d)).astype(np.complex128)
+            data.append(A)
+        return MPO(data, strategy=self.strategy)
+
 
 __all__ = ["MPO", "_mpo_multiply_tensor"]

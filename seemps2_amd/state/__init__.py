@@ -17,6 +17,7 @@ from .canonical_mps import CanonicalMPS
 from .factories import mps_ones, product_state, random_mps, random_uniform_mps
 from .antilinear import AntilinearForm
 from .simplification import SIMPLIFICATION_STRATEGY, combine, simplify, simplify_mps
+from .compose import mps_tensor_product
 
 __all__ = [
     "AntilinearForm",
@@ -34,6 +35,7 @@ __all__ = [
     "Truncation",
     "combine",
     "mps_ones",
+    "mps_tensor_product",
     "product_state",
     "random_mps",
     "random_uniform_mps",

@@ -120,17 +120,25 @@ class MPS(TensorArray):
         raise TypeError(f"Invalid operation - between {type(self)} and {type(state)}")
 
     def __mul__(self, n) -> "MPS":
-        """n * self for a scalar n: scales the first tensor (mps.py:191-212)."""
-        if isinstance(n, (int, float, complex)):
+        """n * self for a scalar n: scales the first tensor; for an MPS n, the
+        site-wise (Hadamard) product with bonds (self, n) (mps.py:191-212)."""
+        if isinstance(n, (int, float, complex, np.number)):
             if n:
                 out = self.copy()
-                out._data[0] = K.scale(n, out._data[0])
+                out._data[0] = K.scale(complex(n), out._data[0])
                 out._error *= abs(n)
                 return out
             return self.zero_state()
+        if isinstance(n, MPS):
+            if n.size != self.size:
+                raise ValueError("MPS product: sizes differ")
+            return MPS([K.site_product(A, B) for A, B in zip(self._data, n._data)])
         raise TypeError(f"Invalid operation * between {type(self)} and {type(n)}")
 
-    __rmul__ = __mul__
+    def __rmul__(self, n) -> "MPS":
+        if isinstance(n, (int, float, complex, np.number)):
+            return self.__mul__(n)
+        raise TypeError(f"Invalid operation * between {type(n)} and {type(self)}")
 
     def __truediv__(self, n) -> "MPS":
         if isinstance(n, (int, float, complex)):

@@ -1,0 +1,165 @@
+"""Closed-form checks of the function-loading input builders, shared by the
+CPU run (kernels replaced by the NumPy test double) and the GPU run.
+
+Every check compares against a dense matrix or vector written from the
+mathematics (shift permutations, stencil sums, Kronecker products, repeated
+Euler steps), not against a restatement of the builder."""
+
+import numpy as np
+
+
+def shift_matrix(L, r, periodic, base=2):
+    n = base ** L
+    S = np.zeros((n, n))
+    for s in range(n):
+        t = s + r
+        if periodic:
+            S[t % n, s] = 1.0
+        elif 0 <= t < n:
+            S[t, s] = 1.0
+    return S
+
+
+def check_interleaving(S):
+    assert list(S.interleaving_permutation([3, 2])) == [0, 3, 1, 4, 2]
+    assert list(S.interleaving_permutation([2, 2, 2])) == [0, 2, 4, 1, 3, 5]
+    assert list(S.interleaving_permutation([1, 3])) == [0, 1, 2, 3]
+
+
+def check_intervals(S):
+    I = S.QuantizedInterval(-1.0, 1.0, 3)
+    assert I.size == 8 and abs(I.step - 0.25) < 1e-15
+    assert np.allclose(I.to_vector(), -1.0 + 0.25 * np.arange(8))
+    J = S.QuantizedInterval(0.0, 1.0, 2, endpoint_right=True)
+    assert np.allclose(J.to_vector(), np.arange(4) / 3)
+    K = S.RegularInterval(0.0, 1.0, 3, endpoint_left=False)
+    assert np.allclose(K.to_vector(), [0.25, 0.5, 0.75])
+    assert abs(I[-1] - 0.75) < 1e-15
+    for bad in (0, -2, 1.5):
+        try:
+            S.QuantizedInterval(0, 1, bad)
+        except ValueError:
+            pass
+        else:
+            raise AssertionError("qubits must be a positive integer")
+
+
+def check_shifts(S, rng):
+    L = 4
+    for periodic in (False, True):
+        shifts = [-3, -1, 0, 2, 5]
+        w = rng.normal(size=len(shifts))
+        O = S.mpo_weighted_shifts(L, w, shifts, periodic=periodic)
+        ref = sum(wk * shift_matrix(L, r, periodic) for wk, r in zip(w, shifts))
+        assert np.abs(O.to_matrix() - ref).max() < 1e-13
+    # range tuple and base 3
+    O = S.mpo_weighted_shifts(3, [1.0, 2.0], (-1, 1), base=3)
+    ref = shift_matrix(3, -1, False, 3) + 2.0 * shift_matrix(3, 0, False, 3)
+    assert np.abs(O.to_matrix() - ref).max() < 1e-13
+    # the bond dimension stays the number of reachable carries
+    assert max(S.mpo_shifts(10, [-1, 0, 1]).bond_dimensions()[1:-1]) <= 3
+
+
+def check_finite_differences(S):
+    n = 6
+    I = S.QuantizedInterval(-1.0, 1.0, n)
+    x = I.to_vector()
+    dx = I.step
+    for order, filt in [(1, 3), (1, 5), (1, 7), (2, 3), (2, 5), (2, 7), (2, 9)]:
+        O = S.finite_differences_mpo(order, I, filter=filt)
+        D = O.to_matrix()
+        from seemps2_amd.analysis.finite_differences import _STENCILS
+
+        w, sh = _STENCILS[(order, filt)]
+        ref = sum(wk * shift_matrix(n, r, False) for wk, r in zip(w, sh)) / dx ** order
+        assert np.abs(D - ref).max() < 1e-9 * np.abs(ref).max()
+        # exact on low-order polynomials away from the boundary
+        f = x ** 2 if order == 2 else 3.0 * x + 1.0
+        df = (D @ f).real
+        inner = slice(8, -8)
+        exact = 2.0 if order == 2 else 3.0
+        assert np.abs(df[inner] - exact).max() < 1e-9
+    # the step is doubled (and the shifts scaled) below tol: 1/64 -> 8/64 >= 0.1
+    O = S.finite_differences_mpo(2, (0.0, 1.0, n), tol=0.1)
+    ref = (shift_matrix(n, -8, False) - 2 * shift_matrix(n, 0, False)
+           + shift_matrix(n, 8, False)) / (8 / 2 ** n) ** 2
+    assert np.abs(O.to_matrix() - ref).max() < 1e-9 * np.abs(ref).max()
+    try:
+        S.finite_differences_mpo(3, I)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("unknown stencil must raise")
+
+
+def check_tridiagonal(S):
+    n = 5
+    a, b, c = 1.5, -0.25, 2.0
+    for closed in (True, False):
+        O = S.tridiagonal_mpo(n, a, b, c, closed=closed)
+        ref = a * np.eye(2 ** n) + b * shift_matrix(n, 1, closed) + c * shift_matrix(n, -1, closed)
+        assert np.abs(O.to_matrix() - ref).max() < 1e-13
+
+
+def check_extend_and_join(S, rng):
+    from oracle import builders as ob
+
+    H = S.MPO(ob.heisenberg_mpo(3))
+    E = H.extend(5, sites=[0, 2, 4])
+    h = H.to_matrix().reshape([2] * 6)
+    # sites 1 and 3 are identities: H acts on qubits (0, 2, 4)
+    dense = np.einsum("ACEace,Bb,Dd->ABCDEabcde", h, np.eye(2), np.eye(2)).reshape(32, 32)
+    assert np.abs(E.to_matrix() - dense).max() < 1e-13
+    F = S.mpo_weighted_shifts(5, [1.0, -2.0], [1, -1])
+    total = 0.5 * E + F
+    ref = 0.5 * E.to_matrix() + F.to_matrix()
+    J = total.join()
+    assert np.abs(J.to_matrix() - ref).max() < 1e-12
+    assert J.bond_dimensions()[1:-1] == [a + b for a, b in zip(E.bond_dimensions()[1:-1],
+                                                                   F.bond_dimensions()[1:-1])]
+    assert np.abs((E - F).to_matrix() - (E.to_matrix() - F.to_matrix())).max() < 1e-12
+
+
+def check_tensor_product(S, rng):
+    from oracle import builders as ob
+
+    a = S.MPS(ob.random_mps([2] * 3, D=2, complex=True, rng=rng).tensors)
+    b = S.MPS(ob.random_mps([2] * 2, D=2, complex=True, rng=rng).tensors)
+    va, vb = a.to_vector(), b.to_vector()
+    pa = S.mps_tensor_product([a, b], "A")
+    assert np.abs(pa.to_vector() - np.kron(va, vb)).max() < 1e-13
+    pb = S.mps_tensor_product([a, b], "B")
+    # "B": joint site order a0 b0 a1 b1 a2
+    ref = np.kron(va, vb).reshape([2] * 5)
+    perm = S.interleaving_permutation([3, 2])
+    ref = ref.transpose(perm).reshape(-1)
+    assert np.abs(pb.to_vector() - ref).max() < 1e-13
+    # site product (Hadamard) of two MPS
+    c = S.MPS(ob.random_mps([2] * 3, D=3, complex=True, rng=rng).tensors)
+    assert np.abs((a * c).to_vector() - va * c.to_vector()).max() < 1e-13
+
+
+def check_euler(S, rng):
+    from oracle import builders as ob
+
+    n = 6
+    I = S.QuantizedInterval(-1.0, 1.0, n)
+    Lap = S.finite_differences_mpo(2, I)
+    x = I.to_vector()
+    f = np.exp(-x ** 2 / 0.1)
+    f /= np.linalg.norm(f)  # from_vector normalises by default
+    psi0 = S.MPS.from_vector(f, [2] * n)
+    st = S.Strategy(tolerance=1e-14, simplification_tolerance=1e-14, max_bond_dimension=64)
+    Lap = Lap.set_strategy(st)
+    dt, steps = 1e-5, 4
+    out = S.euler(Lap, (0.0, dt * steps), psi0, steps=steps, strategy=st)
+    D = Lap.to_matrix()
+    ref = f.astype(complex)
+    for _ in range(steps):
+        ref = ref + dt * (D @ ref)
+    # simplify stops at 2|1 - fidelity| < 1e-14, which bounds the vector error by
+    # ~sqrt(1e-14) per step (2e-8 measured)
+    assert np.linalg.norm(out.to_vector() - ref) < 1e-6 * np.linalg.norm(ref)
+    # callbacks receive every time of the grid
+    ts = S.euler(Lap, [0.0, dt, 3 * dt], psi0, strategy=st, callback=lambda t, s: t)
+    assert np.allclose(ts, [0.0, dt, 3 * dt])

@@ -106,6 +106,13 @@ def embed_block(B, A, l_off, r_off):
     B[l_off:l_off + a, :, r_off:r_off + b] = A
 
 
+def site_product(A, B):
+    A, B = _np(A), _np(B)
+    a, d, b = A.shape
+    c, _, e = B.shape
+    return _t((A[:, None, :, :, None] * B[None, :, :, None, :]).reshape(a * c, d, b * e))
+
+
 def dotc_dev(x, y):
     return _t(np.array([np.vdot(_np(x), _np(y))]))
 
