@@ -53,6 +53,8 @@ def parse():
                     help="skip the learned-rank (randomized SVD) form of the timed sweeps")
     ap.add_argument("--sketch-power-iters", type=int, default=1,
                     help="power iterations of the learned form's randomized SVD")
+    ap.add_argument("--no-simplify", action="store_true",
+                    help="skip the simplify-at-chi comparison of the two forms")
     ap.add_argument("--eig-precision", default="mixed", choices=["fp64", "mixed"],
                     help="eigensolver contractions of the learned form")
     ap.add_argument("--kernel-timing", default=TIMED_KERNELS,
@@ -233,6 +235,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
+    learned_form.net = net
     return dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
                 bond_updates_at_chi=n_chi, seconds_in_chi_updates=round(t_chi, 4),
                 bond_updates_total=n_all, matvecs_per_update=round(matvecs / max(n_all, 1), 2),
@@ -244,6 +247,58 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
                 matvecs=matvecs, matvecs_fp32=mv32,
                 training_spectra=int(X.shape[0]), training_seconds=round(t_train, 3),
                 ms_per_step=round(secs / max(steps, 1) * 1e3, 3))
+
+
+def simplify_form(QF, H, chi, net, sweeps=2):
+    """Variational simplify at chi: H psi (bond 5 chi) back to bond <= chi,
+    from the guess psi, `sweeps` two-site sweeps, plain FP64 against the
+    learned-rank form (same predictor).  Returns both times, the speedup and
+    how far the two results differ."""
+    import numpy as np
+    import torch
+    import seemps2_amd as S
+    from seemps2_amd import kernels as K, learned
+
+    # the canonical DMRG state is the guess; sweeping away from its centre needs
+    # no re-canonicalisation
+    psi = S.CanonicalMPS(list(QF.state), center=QF.state.center, is_canonical=True)
+    direction = +1 if psi.center == 0 else -1
+    Hpsi = H.apply(psi, simplify=False)
+    st = S.SIMPLIFICATION_STRATEGY.replace(max_bond_dimension=chi, max_sweeps=sweeps)
+
+    def run():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = S.simplify(Hpsi, strategy=st, direction=direction,
+                         guess=S.CanonicalMPS(list(psi), center=psi.center, is_canonical=True))
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    run()  # warm-up (workspace growth)
+    a, t_plain = run()
+    lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=1, min_size=64)
+    with learned.learned_truncation(lt):
+        b, t_learned = run()
+
+    def overlap(x, y):
+        return complex(S.scprod(x, y))
+
+    na, nb = np.sqrt(abs(overlap(a, a))), np.sqrt(abs(overlap(b, b)))
+    nh = np.sqrt(abs(overlap(Hpsi, Hpsi)))
+    fa = abs(overlap(a, Hpsi)) / (na * nh)
+    fb = abs(overlap(b, Hpsi)) / (nb * nh)
+    rel = abs(fa - fb) / fa
+    log(f"simplify at chi={chi} ({sweeps} sweeps of H psi, bond {Hpsi.max_bond_dimension()}): "
+        f"plain {t_plain:.3f} s, learned {t_learned:.3f} s (x{t_plain / t_learned:.2f}), "
+        f"fidelity {fa:.15f} vs {fb:.15f} (rel. diff {rel:.2e}), "
+        f"{lt.basis_splits} basis-sketched, {lt.fallbacks} fallbacks of {lt.calls} splits")
+    return dict(workload=f"simplify(H psi) from the chi={chi} DMRG state, {sweeps} sweeps, "
+                         f"N={QF.state.size}", seconds_plain=round(t_plain, 4),
+                seconds_learned=round(t_learned, 4),
+                speedup_vs_plain=round(t_plain / t_learned, 3),
+                fidelity_plain=float(fa), fidelity_learned=float(fb),
+                fidelity_rel_diff=float(f"{rel:.3e}"), splits=lt.calls,
+                basis_splits=lt.basis_splits, fallbacks=lt.fallbacks)
 
 
 def report_kernels(records, secs):
@@ -393,6 +448,9 @@ def main():
     if snapshot is not None and len(spectra.targets) >= 16:
         chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value, E,
                               args.sketch_power_iters, args.eig_precision, args.kernel_timing)
+    simp = None
+    if chosen is not None and not args.no_simplify:
+        simp = simplify_form(QF, H, chi, learned_form.net)
     cpu = None
     if not args.no_cpu_baseline:
         start = advance_to_cap(QF, strategy, direction, chi)
@@ -444,6 +502,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_fp64": plain,
         "chosen_form": chosen,
+        "simplify_at_chi": simp,
     }
     print(json.dumps(line), flush=True)
 

@@ -14,14 +14,19 @@
 //      formed (partial sums over row chunks in different workgroups), an inner
 //      Hermitian Jacobi sweep accumulates Q, and W_pq <- W_pq Q, V_pq <- V_pq Q.
 //      Stops when every normalised off-diagonal |G_ij| / sqrt(G_ii G_jj) seen in
-//      a sweep is below 8 sqrt(len) eps (relative orthogonality).
+//      a sweep is below 32 sqrt(len) eps (relative orthogonality), when it was
+//      already below 1e-9 (quadratic convergence: that sweep's rotations leave
+//      ~1e-18), or when it stalls within 16x of the target.
 //   4. sigma_k = ||w_k||, sorted descending; the W side gives the normalised
 //      singular vectors of the working matrix, V (times Q in the QR path) the
 //      other side.  Exactly-zero singular values get an orthonormal completion.
 //
-// The Jacobi runs as one cooperative persistent kernel: two bounded grid
-// barriers per block step.  A workgroup that waits too long raises an abort
-// flag that every waiter observes, so every wave always exits.
+// The Jacobi runs as one cooperative persistent kernel.  With one work item
+// per workgroup, steps are ordered by per-pair and per-row-chunk counters and
+// one bounded grid barrier per sweep reduces the convergence measure;
+// otherwise two bounded grid barriers per block step.  A workgroup that waits
+// too long raises an abort flag that every waiter observes, so every wave
+// always exits.
 //
 // Replaces _destructive_svd (src/seemps/cython/svd.pxi:17-83) and the
 // truncation rules of src/seemps/cython/truncation.pxi:222-334.
@@ -43,6 +48,9 @@ constexpr int BW = PW / 2;      // block width (columns); PW, GP from jacobi_inn
 constexpr int RT = 32;          // rows per LDS tile
 constexpr int RTP = RT + 1;     // padded tile row length (complex)
 constexpr int RMAX = 16;        // max row chunks per pair
+// Relative off-diagonal measure from which one more sweep is known to suffice
+// (quadratic convergence: the sweep leaves ~1e-18 against a target ~1e-13).
+constexpr double QUAD_STOP = 1e-9;
 constexpr int QRNB = 16;        // QR panel width
 constexpr int QR_MIN_COLS = 17; // use the QR preconditioner from this many columns
 constexpr int MAX_SWEEPS = 60;
@@ -649,8 +657,12 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
       if (blockIdx.x == 0 && tid == 0 && sweep < 64) w.ctrl->offhist[sweep] = off_sweep;
       // Converged at the target, or stalled within 16x of it: near convergence
       // Jacobi reduces the measure quadratically, so a sweep that gains less than
-      // 4x there is sitting on the rounding floor.
-      if (off_sweep <= tol || (off_sweep <= 16.0 * tol && off_sweep > 0.25 * off_prev)) {
+      // 4x there is sitting on the rounding floor.  A sweep whose measure (taken
+      // before each pair's rotation) was already below QUAD_STOP leaves an
+      // off-diagonal of order QUAD_STOP^2, far below tol: the sweep that would
+      // only confirm it is skipped.
+      if (off_sweep <= tol || off_sweep <= QUAD_STOP ||
+          (off_sweep <= 16.0 * tol && off_sweep > 0.25 * off_prev)) {
         converged = 1;
         ++sweep;
         break;
