@@ -99,6 +99,33 @@ int seemps_zgemm_prec(int precision, int opA, int opB, int64_t M, int64_t N, int
 int seemps_zsite_product(int64_t a, int64_t d, int64_t b, int64_t c, int64_t e,
                          const void* A, const void* B, void* out, void* stream);
 
+/* Grouped complex GEMM (no counterpart in the reference: the batching of many
+ * MPS of uneven size).  C_p = op(A) op(B) for every problem p of desc (a
+ * device array of nprob SeempsGemmDesc, row-major operands as in
+ * seemps_zgemm, alpha = 1, beta = 0); tile_prob (device, ntiles entries)
+ * lists the problem of every output tile, each problem's tiles contiguous
+ * from desc[p].tile0 in row-major tile order with desc[p].tiles_n tiles per
+ * tile row (seemps_grouped_tiles gives a problem's tile count).  tile_cfg:
+ * SEEMPS_TILE_64x64 (256 threads) or SEEMPS_TILE_32x32 (64 threads).  Op
+ * pairs: (N,N), (H,N), (N,T), (T,N), (C,N), (N,H).  Allocates nothing and
+ * does not synchronise, so it can be captured in a HIP graph. */
+#define SEEMPS_TILE_64x64 0
+#define SEEMPS_TILE_32x32 1
+typedef struct {
+  int64_t M, N, K;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+  int64_t tiles_n;
+  int64_t tile0;
+} SeempsGemmDesc;
+int64_t seemps_grouped_tiles(int tile_cfg, int64_t M, int64_t N);
+int seemps_zgemm_grouped(int opA, int opB, int tile_cfg, int64_t nprob, const void* desc,
+                         const int32_t* tile_prob, int64_t ntiles, void* stream);
+
 /* Local-operator contraction, complex128:
  *   out[p, j1, q, j2] = sum_i in[p, i, q] * Mop[i, j1, j2]   (+ out if accumulate)
  * with in[P, I, Q], Mop[I, J1, J2], out[P, J1, Q, J2] contiguous, 64*I + I*J1*J2 <= 3072.  Replaces the small-index steps of _mpo_multiply_tensor

@@ -45,6 +45,8 @@ _SIGNATURES = {
          _P],
     ),
     "seemps_zsite_product": (c_int, [_I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "seemps_grouped_tiles": (_I, [c_int, _I, _I]),
+    "seemps_zgemm_grouped": (c_int, [c_int, c_int, c_int, _I, _P, _P, _I, _P]),
     "seemps_zlocal": (c_int, [_I, _I, _I, _I, _I, _P, _P, _P, c_int, _P]),
     "seemps_zaxpby": (c_int, [_I, _D, _D, _P, c_int, _D, _D, _P, _P]),
     "seemps_zscale2d": (c_int, [_I, _I, _P, _P, _P, _I, _P, _I, _P]),

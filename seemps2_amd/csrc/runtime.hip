@@ -109,6 +109,10 @@ int drain_pending() {
 TimedLaunch::TimedLaunch(const char* name, hipStream_t stream, double flops)
     : slot_(-1), start_(nullptr), stream_(stream), flops_(flops) {
   if (!g_timing_on.load(std::memory_order_relaxed)) return;
+  // no event nodes inside a HIP graph being captured
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+    return;
   std::lock_guard<std::mutex> lock(g_timing_mu);
   const int slot = timing_slot(name);
   if (slot < 0) return;

@@ -137,6 +137,34 @@ def test_simplify_sum_parity(S, rng):
     assert rel(out.to_vector(), ref.to_vector()) < 1e-9
 
 
+def test_simplify_uneven_sums_parity(S):
+    """Many-state sums of uneven length and bond dimension, complex64 and
+    complex128 inputs (promoted to complex128 as the reference promotes inside
+    its SVD/GEMM): simplify(psi_k + phi_k) matches the oracle (same bond
+    dimensions, states within 1e-9) and the exact sum (fidelity)."""
+    rng = np.random.default_rng(404)
+    for k in range(6):
+        N = int(rng.integers(10, 15))
+        Da, Db = int(rng.integers(4, 25)), int(rng.integers(4, 25))
+        a = ob.random_mps([2] * N, D=Da, complex=True, rng=rng)
+        b = ob.random_mps([2] * N, D=Db, complex=True, rng=rng)
+        if k % 2:
+            a = oa.OMPS([A.astype(np.complex64).astype(np.complex128) for A in a.tensors], 0.0)
+            da = S.MPS([A.astype(np.complex64) for A in a.tensors])
+        else:
+            da = dev_mps(S, a)
+        st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+        ost = oc.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+        ref = oa.simplify(oa.OMPSSum([1.0, 1.0], [a, b]), ost)
+        out = S.simplify(S.MPSSum([1.0, 1.0], [da, dev_mps(S, b)]), st)
+        assert out.bond_dimensions() == ref.bond_dimensions(), (k, N, Da, Db)
+        assert rel(out.to_vector(), ref.to_vector()) < 1e-9
+        exact = a.to_vector() + b.to_vector()
+        v = out.to_vector()
+        fid = abs(np.vdot(v, exact)) / (np.linalg.norm(v) * np.linalg.norm(exact))
+        assert 1 - fid < 1e-7
+
+
 def test_dmrg_parity_small(S):
     N = 10
     rng = np.random.default_rng(3)
