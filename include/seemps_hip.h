@@ -228,7 +228,7 @@ int seemps_zpermute(int ndim, const int64_t* dims, const int* perm, const void* 
  * restarts from the Ritz vector, started from v0; stops when the Ritz residual
  * |beta_k y_k| <= tol * max(eps^(2/3), |theta|) (ARPACK's test; tol = 0 means
  * machine epsilon), tested at the end of each cycle (k = ncv) and, for
- * operators of dimension >= 8192, also after steps k = 8, 12, 16, ... so a warm
+ * operators of dimension >= 8192, also after steps k = 4, 6, 8, ... so a warm
  * start stops before ncv products.  The Krylov recurrence, the tridiagonal eigensolve (FP64
  * bisection + inverse iteration) and the Ritz vector stay on the device; the
  * stream is synchronised once per restart.  evec receives the unit-norm

@@ -667,12 +667,12 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
   int mv_prec = prec == SEEMPS_PREC_FP32 ? SEEMPS_PREC_FP32 : SEEMPS_PREC_FP64;
   const double fp32_floor = 1e-5, fp32_start = 1e-4;
   double last_resid = HUGE_VAL;
-  // In FP64 cycles the Ritz residual is also tested after steps 8, 12, 16, ...
+  // In FP64 cycles the Ritz residual is also tested after steps 4, 6, 8, ...
   // (one single-block tridiagonal solve and a synchronisation each), so that a
   // warm start stops well before ncv products; only for operators large enough
   // that a product outweighs the check.
   const bool early = n >= 8192;
-  const int64_t first_check = 8, check_every = 4;
+  const int64_t first_check = 4, check_every = 2;
   for (cycle = 0; cycle < maxiter; ++cycle) {
     hipLaunchKernelGGL(lz_reset, dim3(1), dim3(64), 0, st_, w.sc);
     int64_t kd_eff = m;
