@@ -536,6 +536,31 @@ __device__ void signal_set(unsigned int* c, unsigned int v) {
   }
 }
 
+// Work item (pair p, row chunk r) of workgroup b when every item has its own
+// workgroup.  Workgroups are dispatched round-robin over the XCDs (b mod 8), so
+// the items are laid out XCD by XCD: each XCD takes whole pairs first (their
+// partial Grams then meet in that XCD's L2 instead of crossing XCDs through
+// memory), and the pairs left over fill the remaining slots.  A bijection of
+// [0, G) onto the items for every G = P R.
+constexpr int64_t NXCD = 8;
+__device__ inline void local_item(int64_t b, int64_t G, int64_t R, int64_t* p, int64_t* r) {
+  if (G % NXCD) {
+    *p = b / R;
+    *r = b % R;
+    return;
+  }
+  const int64_t per = G / NXCD, x = b % NXCD, j = b / NXCD;
+  const int64_t f = per / R;  // whole pairs per XCD
+  if (j < f * R) {
+    *p = x * f + j / R;
+    *r = j % R;
+    return;
+  }
+  const int64_t q = x * (per - f * R) + (j - f * R);
+  *p = NXCD * f + q / R;
+  *r = q % R;
+}
+
 // One-sided block Jacobi sweeps.  Work item (p, r) = row chunk r of pair p.
 // LOCAL: one item per workgroup (gridDim.x == P R); the steps are ordered by
 // local counters only -- the R workgroups of a pair meet after their partial
@@ -582,7 +607,8 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
       double2* part = w.part + (g & 1) * part_stride;
       if (prof) t0 = now_ticks();
       if (LOCAL) {
-        const int64_t p = blockIdx.x / s.R, r = blockIdx.x % s.R;
+        int64_t p, r;
+        local_item(blockIdx.x, gridDim.x, s.R, &p, &r);
         int64_t bi, bj;
         pair_blocks(s.nb, k, p, &bi, &bj);
         if (g > 0) {

@@ -261,11 +261,11 @@ def test_eigensolve_early_exit_in_cycle(K, rng):
                         matvec=lambda x: oal.two_site_matvec(L, H12, R, x.reshape(v.shape)).ravel())
     ref = eigsh(op, k=1, which="SA", tol=1e-13, ncv=40, maxiter=10000)[0][0]
     assert abs(lam0 - ref) < 1e-9 * abs(ref)
-    # started from the converged vector: the first in-cycle test (after 4
-    # products) stops it (random environments leave a tiny spectral gap, so
-    # only a start this close converges within one cycle)
+    # started from the converged vector: an in-cycle test (after 4, 6, 8, ...
+    # products) stops it well before ncv (random environments leave a tiny
+    # spectral gap, so only a start this close converges within one cycle)
     lam1, _, st1 = K.eigensolve(Ld, Hm, Rd, vec0, d, d, ncv=20, tol=1e-10)
-    assert st1["restarts"] == 1 and st1["matvecs"] == 4
+    assert st1["restarts"] == 1 and st1["matvecs"] <= 12 and st1["matvecs"] % 2 == 0
     assert abs(lam1 - ref) < 1e-9 * abs(ref)
     # below the size threshold the same start runs the whole cycle
     small = slice(0, 40)
