@@ -53,6 +53,11 @@ def parse():
                     help="skip the learned-rank (randomized SVD) form of the timed sweeps")
     ap.add_argument("--sketch-power-iters", type=int, default=1,
                     help="power iterations of the learned form's randomized SVD")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="grow to chi, run one split in each form and a calibration copy, exit "
+                         "(the program profiled by the rocprofv3 --pmc passes)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="reduce the two --pmc passes of --pmc-workload (no GPU work)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-overlap measurement of many uneven states")
     ap.add_argument("--no-simplify", action="store_true",
@@ -115,6 +120,83 @@ def rocprof_summary(directory, bench_json=None, top=25):
                            f"trace): average {avg:.1f} us; bench.py's HIP-event average "
                            f"{roof.get('avg_launch_us')} us.")
     return "\n".join(out)
+
+
+PMC_COPY_ELEMS = 4096 * 4096   # complex128 elements of the calibration copy (256 MiB)
+PMC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "svd_jacobi_dmrg_chi256_pmc_traffic.md")
+
+
+def pmc_workload(QF, H, strategy, chi, direction):
+    """Dispatches for the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE):
+    one chi x chi bond split in the plain form and one in the basis-sketched
+    form (the dominant svd_jacobi launches of each), then a plain copy of
+    PMC_COPY_ELEMS complex128 elements that calibrates the counters."""
+    import importlib
+    import torch
+    from seemps2_amd import kernels as K
+
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
+    start = advance_to_cap(QF, strategy, direction, chi)
+    _, AB = D._diagonalize_two_site(QF, start, 1e-10)
+    a, d1, d2, b = AB.shape
+    theta = AB.reshape(a * d1, d2 * b)
+    m, n = theta.shape
+    if direction > 0:
+        basis, side, k = QF.state[start + 1], 0, QF.state[start + 1].shape[0]
+        bm = basis.reshape(k, n)
+    else:
+        basis, side, k = QF.state[start], 1, QF.state[start].shape[2]
+        bm = basis.reshape(m, k)
+    K.svd_split(theta, m, n, side, strategy.method, strategy.tolerance, chi)
+    K.svd_split_basis(theta, m, n, side, bm, k, 16, side, strategy.method, strategy.tolerance,
+                      chi, seed=3)
+    x = K.empty(PMC_COPY_ELEMS)
+    y = K.empty(PMC_COPY_ELEMS)
+    K.fill_zero(x)
+    K.scale2d(x.view(4096, 4096), 4096, 4096, 4096, out=y.view(4096, 4096))
+    torch.cuda.synchronize()
+    log(f"pmc workload: split of bond {start} ({m} x {n}), basis side {side}, k = {k}")
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """Traffic of the svd_jacobi launches of --pmc-workload from the two passes,
+    each counter scaled by what it reads for the calibration copy."""
+    import csv
+    import glob
+
+    def per_dispatch(directory):
+        path = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
+                         recursive=True)[0]
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            key = (int(r["Dispatch_Id"]), kernel_base_name(r["Kernel_Name"]))
+            acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+        return sorted(acc.items())
+
+    f, w = per_dispatch(fetch_dir), per_dispatch(write_dir)
+    copy_kib = PMC_COPY_ELEMS * 16 / 1024
+    cf = copy_kib / [v for (_, k), v in f if k == "zscale2d_kernel"][-1]
+    cw = copy_kib / [v for (_, k), v in w if k == "zscale2d_kernel"][-1]
+    jf = [v for (_, k), v in f if k == "svd_jacobi"][-2:]
+    jw = [v for (_, k), v in w if k == "svd_jacobi"][-2:]
+    forms = {}
+    for name, fk, wk in zip(("plain", "chosen"), jf, jw):
+        forms[name] = {"fetch_bytes": round(fk * cf * 1024), "write_bytes": round(wk * cw * 1024),
+                       "traffic_bytes": round((fk * cf + wk * cw) * 1024)}
+    out = {"kernel": "svd_jacobi", "fetch_scale": round(cf, 4), "write_scale": round(cw, 4),
+           "forms": forms}
+    return out
+
+
+def read_traffic(form):
+    """HBM bytes per svd_jacobi launch of `form` from the committed PMC summary."""
+    try:
+        text = open(PMC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["forms"][form]["traffic_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
 
 
 def grow(QF, strategy, chi, log_every=True, max_sweeps=12):
@@ -237,6 +319,8 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
+        if roof["kernel"] == "svd_jacobi":
+            roof["traffic"] = read_traffic("chosen")
     learned_form.net = net
     return dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
                 bond_updates_at_chi=n_chi, seconds_in_chi_updates=round(t_chi, 4),
@@ -454,6 +538,9 @@ def main():
     if args.rocprof_summary:
         print(rocprof_summary(args.rocprof_summary, args.bench_json))
         return
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the DMRG sweep is "
               "sequential along the chain and this build measures one MI355X.", file=sys.stderr)
@@ -476,6 +563,9 @@ def main():
     t0 = time.perf_counter()
     direction = grow(QF, strategy, chi)
     log(f"grown to chi={max(QF.state.bond_dimensions())} in {time.perf_counter() - t0:.1f} s")
+    if args.pmc_workload:
+        pmc_workload(QF, H, strategy, chi, direction)
+        return
     from seemps2_amd import learned
 
     spectra = learned.SpectrumLog(min_size=64)
@@ -501,6 +591,8 @@ def main():
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
+        if roof["kernel"] == "svd_jacobi":
+            roof["traffic"] = read_traffic("plain")
     chosen = None
     if snapshot is not None and len(spectra.targets) >= 16:
         chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value, E,

@@ -314,10 +314,21 @@ __global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __r
     Q[i * GP + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
+  // Diagonal entries below 1e-24 of the largest are noise for the features
+  // (clamped at a ratio of 1e-16 in eigenvalue): they take no rotations, so
+  // the sweeps stop once the resolved part has converged.
+  __shared__ double s_tiny;
+  if (tid == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < PW; ++i) mx = fmax(mx, A[i * GP + i].x);
+    s_tiny = 1e-24 * mx;
+  }
+  __syncthreads();
+  const double tiny = s_tiny;
   double2* cur = A;
   for (int sw = 0; sw < 12; ++sw) {
     int nact = 0;
-    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-12, 0.0, 1, &nact);
+    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-12, tiny, 1, &nact);
     __syncthreads();  // the next call rewrites the round flags
     if (nact == 0) break;  // a sweep without rotations: converged (uniform)
   }
