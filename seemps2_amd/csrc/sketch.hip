@@ -68,6 +68,15 @@ __global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __
   const double tail = resid2->x > 0.0 ? resid2->x : 0.0;
   int64_t D = l;
   int fallback = 0;
+  // A sketch decides a capped truncation only when what it did not see cannot
+  // matter: the unsketched weight is negligible against the matrix, or far
+  // below the smallest kept value (so no unseen direction competes with the
+  // kept ones).  Otherwise (flat spectra, e.g. random circuits) the full SVD
+  // decides.
+  auto capped_ok = [&](int64_t Dk) {
+    const double sD = s[Dk - 1];
+    return tail <= 1e-14 * total || tail <= 1e-3 * sD * sD;
+  };
   if (method == SEEMPS_TRUNC_NONE) {
     fallback = 1;
   } else if (method == SEEMPS_TRUNC_REL_NORM2) {
@@ -75,10 +84,12 @@ __global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __
     if (tail > max_error) {
       // the tolerance alone would keep more than l values; the sketch still
       // decides when the bond cap binds below l
-      if (max_bond < l)
+      if (max_bond < l) {
         D = max_bond;
-      else
+        if (!capped_ok(D)) fallback = 1;
+      } else {
         fallback = 1;
+      }
     } else {
       double acc = tail;
       int64_t k = 0;
@@ -104,6 +115,8 @@ __global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __
     }
     // the rule would look past the sketch: only the full spectrum can decide
     if (!found && D == l && l < cap_d) fallback = 1;
+    // the cap decided below the sketch size: certify as above
+    if (!found && D == cap_d && D < l && !capped_ok(D)) fallback = 1;
   }
   if (D > max_bond) D = max_bond;
   if (D < 1) D = 1;

@@ -2,5 +2,6 @@
 integrator of ``seemps.evolution``)."""
 
 from .euler import euler, ode_solver
+from .trotter import PairwiseUnitaries, Trotter2ndOrder, Trotter3rdOrder
 
-__all__ = ["euler", "ode_solver"]
+__all__ = ["PairwiseUnitaries", "Trotter2ndOrder", "Trotter3rdOrder", "euler", "ode_solver"]

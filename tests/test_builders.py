@@ -52,3 +52,11 @@ def test_tensor_product(S, rng):
 
 def test_euler(S, rng):
     bc.check_euler(S, rng)
+
+
+def test_trotter(S, rng):
+    bc.check_trotter(S, rng)
+
+
+def test_gate_layer(S, rng):
+    bc.check_gate_layer(S, rng)

@@ -301,6 +301,15 @@ def _decaying(rng, m, n, rate):
     return (U0 * s0) @ V0.conj().T, s0
 
 
+def _capped_certification(s, D, l):
+    """(certified with a 10x margin, refused) for a sketch of width l deciding a
+    cap D: the unsketched weight (at least the true tail beyond l) must be
+    <= max(1e-14 ||A||^2, 1e-3 sigma_D^2)."""
+    tail = float(np.sum(s[l:] ** 2))
+    thr = max(1e-14 * float(np.sum(s ** 2)), 1e-3 * float(s[D - 1]) ** 2)
+    return 10 * tail <= thr, tail > thr
+
+
 def _sketch_bound(s, D, l, q):
     """Randomized-range accuracy of the kept rank-D part (relative to ||A||):
     ~ sigma_l (sigma_l / sigma_(D-1))^(2q), with a safety factor of 10."""
@@ -326,6 +335,12 @@ def test_svd_split_sketch_matches_full(K, rng, m, n, rate, tol, mb):
         Ad, Bd, D, err2, _, fb = K.svd_split_sketch(K.as_dev(A), m, n, mode, st.method,
                                                     st.tolerance, st.max_bond_dimension, l,
                                                     power_iters=2, seed=7)
+        if Dref == mb:
+            certified, refused = _capped_certification(sv, Dref, l)
+            assert fb if refused else (not fb if certified else True)
+            if fb:
+                assert D == Dref  # the full split decided
+                continue
         assert not fb
         assert D == Dref
         # the unsketched weight is ||A||^2 - sum s^2: an estimate, good to ~1e-3 here
@@ -350,9 +365,14 @@ def test_svd_split_sketch_matches_full(K, rng, m, n, rate, tol, mb):
                                                 cap, min(cap + 32, min(m, n) - 1), power_iters=2,
                                                 seed=3)
     lc = min(cap + 32, min(m, n) - 1)
-    assert not fb and D == cap == Ar.shape[-1]
+    assert D == cap == Ar.shape[-1]
+    certified, refused = _capped_certification(sv, cap, lc)
+    if refused:
+        assert fb  # the unseen weight may compete with the kept values: full SVD
+    elif certified:
+        assert not fb
     assert rel(K.to_numpy(Ad) @ K.to_numpy(Bd), Ar.reshape(m, -1) @ Br.reshape(cap, -1)) < \
-        _sketch_bound(sv, cap, lc, 2)
+        (1e-12 if fb else _sketch_bound(sv, cap, lc, 2))
 
 
 def _in_span(rng, m, n, k, side, rate=0.05):
