@@ -7,6 +7,8 @@ quadratic.py:103,198-201), built once per bond on the device.
 
 from __future__ import annotations
 
+import numpy as np
+
 from .. import kernels as K
 from ..state import CanonicalMPS, MPS, Strategy
 from ..state.environments import (
@@ -60,6 +62,31 @@ class QuadraticForm:
         """H_eff(i, i+1) v on a device tensor v[b, k, l, f] (quadratic.py:204-214)."""
         L, Hm, R = self.two_site_block(i)
         return K.two_site_matvec(L, Hm, R, K.as_dev(v), self.O[i].shape[1], self.O[i + 1].shape[1])
+
+    def one_site_Hm(self, i: int):
+        """Hm for site i alone: the MPO tensor paired with an identity tensor of
+        its right bond (a one-site operator in the two-site layout, d2 = 1)."""
+        key = ("one", i)
+        Hm = self._hm.get(key)
+        if Hm is None:
+            w = self.O[i].shape[3]
+            eye = K.as_dev(np.eye(w, dtype=np.complex128).reshape(w, 1, 1, w))
+            Hm = self._hm[key] = K.two_site_operator(self.O[i], eye)
+        return Hm
+
+    def one_site_matvec(self, i: int, v):
+        """H_eff(i) v on a device tensor v[b, k, f] (quadratic.py:239-271): the
+        environments left of site i and right of site i around the MPO tensor."""
+        v = K.as_dev(v)
+        b, d, f = v.shape
+        out = K.two_site_matvec(self.left_env[i], self.one_site_Hm(i), self.right_env[i],
+                                v.reshape(b, d, 1, f), d, 1)
+        return out.view(b, d, f)
+
+    def two_site_operator_matvec(self, i: int, v):
+        """H_eff(i, i+1) v keeping v's shape."""
+        v = K.as_dev(v)
+        return self.two_site_matvec(i, v).view(v.shape)
 
     def move_right(self, i: int):
         j = self.site

@@ -60,3 +60,11 @@ def test_trotter(S, rng):
 
 def test_gate_layer(S, rng):
     bc.check_gate_layer(S, rng)
+
+
+def test_expmv(S, rng):
+    bc.check_expmv(S, rng)
+
+
+def test_tdvp(S, rng):
+    bc.check_tdvp(S, rng)

@@ -157,3 +157,11 @@ def K_rand(rng, n):
     from seemps2_amd import kernels as K
 
     return K.as_dev(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+
+
+def test_expmv(S, rng):
+    bc.check_expmv(S, rng)
+
+
+def test_tdvp(S, rng):
+    bc.check_tdvp(S, rng)
