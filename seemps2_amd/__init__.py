@@ -45,6 +45,7 @@ from .analysis import (
 )
 from .register import mpo_shifts, mpo_weighted_shifts
 from .evolution import euler
+from .solve import dmrg_solve
 from .hamiltonians import (
     ConstantNNHamiltonian,
     ConstantTIHamiltonian,
@@ -59,6 +60,7 @@ cython = core
 _sys.modules[__name__ + ".cython"] = core
 
 __all__ = [
+    "dmrg_solve",
     "MPOSum",
     "QuantizedInterval",
     "RegularInterval",

@@ -265,3 +265,37 @@ def check_tdvp(S, rng):
     w = out.to_vector()
     w /= np.linalg.norm(w)
     assert np.vdot(w, Hd @ w).real < E0 - 1e-3
+
+
+def check_dmrg_solve(S, rng):
+    """(H + 6 I) x = b for the XXZ chain (positive definite): every Krylov
+    method reaches the dense solution."""
+    from oracle import builders as ob
+
+    N = 8
+    Hn = S.XXZHamiltonian(N, 1.0, 0.5)
+    dense = lambda M: np.asarray(M.todense()) if hasattr(M, "todense") else np.asarray(M)
+    Ad = dense(Hn.to_matrix()) + 6.0 * np.eye(2 ** N)
+    Id = S.MPO([np.eye(2).reshape(1, 2, 2, 1)] * N)
+    A = (Hn.to_mpo() + Id * 6.0).join()
+    assert np.abs(A.to_matrix() - Ad).max() < 1e-12
+    bm = ob.random_mps([2] * N, D=4, complex=True, rng=rng)
+    bv = bm.to_vector()
+    x_ref = np.linalg.solve(Ad, bv)
+    st = S.Strategy(tolerance=1e-14, max_bond_dimension=64)
+    for method in ("cg", "bicgstab", "gmres"):
+        x, res = S.dmrg_solve(A, S.MPS(bm.tensors), rtol=1e-10, strategy=st, method=method)
+        assert np.linalg.norm(x.to_vector() - x_ref) < 1e-7 * np.linalg.norm(x_ref), method
+        # the residual is ||Ax||^2 - 2 Re<Ax|b> + ||b||^2 (as in the reference):
+        # cancellation limits it to ~1e-8 ||b||
+        assert res <= 1e-6 * np.linalg.norm(bv)
+    x, res = S.dmrg_solve(A, S.MPS(bm.tensors), rtol=1e-10, strategy=st,
+                          compute_residuals=False)
+    assert res is None
+    assert np.linalg.norm(x.to_vector() - x_ref) < 1e-6 * np.linalg.norm(x_ref)
+    try:
+        S.dmrg_solve(A, S.MPS(bm.tensors), method="nope")
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("unknown method must raise")

@@ -165,3 +165,7 @@ def test_expmv(S, rng):
 
 def test_tdvp(S, rng):
     bc.check_tdvp(S, rng)
+
+
+def test_dmrg_solve(S, rng):
+    bc.check_dmrg_solve(S, rng)
