@@ -81,6 +81,16 @@ int seemps_zgemm(int opA, int opB, int64_t M, int64_t N, int64_t K,
                  void* C, int64_t ldc, int64_t strideC,
                  int64_t batch, void* stream);
 
+/* Single (batch 1) seemps_zgemm that splits K over a caller workspace when the
+ * output has too few tiles to fill the device (slices reduced in a fixed
+ * order: deterministic).  seemps_zgemm_workspace gives the bytes (0 = no
+ * split pays; any work pointer is then accepted). */
+int64_t seemps_zgemm_workspace(int64_t M, int64_t N, int64_t K);
+int seemps_zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K,
+                    double alpha_re, double alpha_im, const void* A, int64_t lda,
+                    const void* B, int64_t ldb, double beta_re, double beta_im,
+                    void* C, int64_t ldc, void* work, int64_t work_bytes, void* stream);
+
 /* seemps_zgemm with the product in `precision` (SEEMPS_PREC_FP64 or
  * SEEMPS_PREC_FP32).  No counterpart in the reference: the FP32 form is the
  * contraction inside the mixed-precision eigensolver. */

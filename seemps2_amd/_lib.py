@@ -39,6 +39,11 @@ _SIGNATURES = {
         c_int,
         [c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _I, _I, _I, _P],
     ),
+    "seemps_zgemm_workspace": (_I, [_I, _I, _I]),
+    "seemps_zgemm_ws": (
+        c_int,
+        [c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
+    ),
     "seemps_zgemm_prec": (
         c_int,
         [c_int, c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _I, _I, _I,

@@ -367,3 +367,24 @@ extern "C" int seemps_zgemm_prec(int precision, int opA, int opB, int64_t M, int
                        ldb, strideB, make_double2(beta_re, beta_im), C, ldc, strideC, batch,
                        (hipStream_t)stream, precision);
 }
+
+extern "C" int64_t seemps_zgemm_workspace(int64_t M, int64_t N, int64_t K) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  return seemps::splitk_workspace(M, N, K) * 16;
+}
+
+extern "C" int seemps_zgemm_ws(int opA, int opB, int64_t M, int64_t N, int64_t K, double alpha_re,
+                               double alpha_im, const void* A, int64_t lda, const void* B,
+                               int64_t ldb, double beta_re, double beta_im, void* C, int64_t ldc,
+                               void* work, int64_t work_bytes, void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(opA >= 0 && opA <= 3 && opB >= 0 && opB <= 3, "zgemm_ws: invalid op flags");
+  SEEMPS_CHECK_ARG(M >= 0 && N >= 0 && K >= 0 && work_bytes >= 0, "zgemm_ws: negative size");
+  if (M == 0 || N == 0) return SEEMPS_OK;
+  if (K == 0)
+    return zgemm(opA, opB, M, N, K, make_double2(alpha_re, alpha_im), A, lda, 0, B, ldb, 0,
+                 make_double2(beta_re, beta_im), C, ldc, 0, 1, (hipStream_t)stream);
+  return zgemm_ws(opA, opB, M, N, K, make_double2(alpha_re, alpha_im), A, lda, B, ldb,
+                  make_double2(beta_re, beta_im), C, ldc, (double2*)work, work_bytes / 16,
+                  (hipStream_t)stream);
+}

@@ -71,6 +71,15 @@ def gemm(A: torch.Tensor, B: torch.Tensor, opA: int = 0, opB: int = 0, alpha=1.0
         C = empty(M, N)
         beta = 0.0
     alpha, beta = complex(alpha), complex(beta)
+    if precision == "fp64":
+        # split K over the workspace when the output has too few tiles
+        wb = lib.seemps_zgemm_workspace(M, N, K)
+        if wb > 0:
+            wp, wn = workspace.get(wb)
+            check(lib.seemps_zgemm_ws(opA, opB, M, N, K, alpha.real, alpha.imag, ptr(A),
+                                      A.shape[1], ptr(B), B.shape[1], beta.real, beta.imag,
+                                      ptr(C), N, wp, wn, stream()))
+            return C
     check(lib.seemps_zgemm_prec(PRECISIONS[precision], opA, opB, M, N, K, alpha.real,
                                 alpha.imag, ptr(A), A.shape[1], 0, ptr(B), B.shape[1], 0,
                                 beta.real, beta.imag, ptr(C), N, 0, 1, stream()))

@@ -15,6 +15,13 @@ class AntilinearForm:
         if bra.size != ket.size:
             raise ValueError("bra and ket must have the same size")
         size = bra.size
+        # a negative centre counts from the end, as the callers' start = -1 of a
+        # right-to-left simplify means (antilinear.py:38-57 would build the
+        # environments for a literal -1 and fail on the first two-site tensor)
+        if center < 0:
+            center += size
+        if not 0 <= center < size:
+            raise IndexError("AntilinearForm: centre out of range")
         rho = _begin_environment()
         R = [rho] * size
         for i in range(size - 1, center, -1):

@@ -137,6 +137,18 @@ def test_simplify_sum_parity(S, rng):
     assert rel(out.to_vector(), ref.to_vector()) < 1e-9
 
 
+def test_simplify_right_to_left_parity(S, rng):
+    N = 10
+    a = ob.random_mps([2] * N, D=7, complex=True, rng=rng)
+    b = ob.random_mps([2] * N, D=5, complex=True, rng=rng)
+    st = S.Strategy(tolerance=1e-9, simplification_tolerance=1e-9, max_bond_dimension=9)
+    ost = oc.Strategy(tolerance=1e-9, simplification_tolerance=1e-9, max_bond_dimension=9)
+    ref = oa.simplify(oa.OMPSSum([1.0, -0.3j], [a, b]), ost, direction=-1)
+    out = S.simplify(S.MPSSum([1.0, -0.3j], [dev_mps(S, a), dev_mps(S, b)]), st, direction=-1)
+    assert out.bond_dimensions() == ref.bond_dimensions()
+    assert rel(out.to_vector(), ref.to_vector()) < 1e-9
+
+
 def test_simplify_uneven_sums_parity(S):
     """Many-state sums of uneven length and bond dimension, complex64 and
     complex128 inputs (promoted to complex128 as the reference promotes inside

@@ -42,6 +42,23 @@ def test_zgemm_ops(K, rng, opA, opB, M, N, K_):
     assert rel(K.to_numpy(out), ref) < 1e-14
 
 
+@pytest.mark.parametrize("opA,opB", [(0, 0), (2, 0), (0, 1), (3, 2), (1, 3)])
+@pytest.mark.parametrize("M,N,K_", [(64, 48, 2048), (512, 512, 1280), (5, 1, 9000)])
+def test_zgemm_splitk_through_python(K, rng, opA, opB, M, N, K_):
+    """Shapes with few output tiles and long K take the split-K path of the
+    Python-level gemm (alpha, beta applied in the slice reduction)."""
+    from seemps2_amd._lib import lib
+
+    assert lib.seemps_zgemm_workspace(M, N, K_) > 0
+    A = crandn(rng, *((K_, M) if opA in (1, 2) else (M, K_)))
+    B = crandn(rng, *((N, K_) if opB in (1, 2) else (K_, N)))
+    C = crandn(rng, M, N)
+    alpha, beta = 0.3 + 0.4j, -1.1 + 0.2j
+    ref = alpha * OPS[opA](A) @ OPS[opB](B) + beta * C
+    out = K.gemm(K.as_dev(A), K.as_dev(B), opA, opB, alpha, beta, K.as_dev(C))
+    assert rel(K.to_numpy(out), ref) < 1e-13
+
+
 @pytest.mark.parametrize("opA", [0, 1, 2, 3])
 @pytest.mark.parametrize("opB", [0, 3, 1])
 @pytest.mark.parametrize("M,N,K_", [(37, 53, 29), (130, 70, 200)])

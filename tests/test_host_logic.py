@@ -58,6 +58,24 @@ def test_apply_simplify(S, rng):
     assert abs(S.MPO(H).expectation(dev(S, a)) - e) < 1e-12 * abs(e)
 
 
+def test_simplify_right_to_left(S, rng):
+    """simplify(direction=-1) starts at the last site (the reference passes
+    start = -1 to AntilinearForm, meant as the last site)."""
+    N = 8
+    a = ob.random_mps([2] * N, D=6, complex=True, rng=rng)
+    b = ob.random_mps([2] * N, D=5, complex=True, rng=rng)
+    ost = oc.Strategy(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=7)
+    st = S.Strategy(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=7)
+    r = oa.simplify(oa.OMPSSum([1.0, 0.5], [a, b]), ost, direction=-1)
+    o = S.simplify(S.MPSSum([1.0, 0.5], [dev(S, a), dev(S, b)]), st, direction=-1)
+    assert o.bond_dimensions() == r.bond_dimensions()
+    assert np.allclose(o.to_vector(), r.to_vector(), atol=1e-10)
+    r1 = oa.simplify(a, ost, direction=-1)
+    o1 = S.simplify(dev(S, a), st, direction=-1)
+    assert o1.bond_dimensions() == r1.bond_dimensions()
+    assert np.allclose(o1.to_vector(), r1.to_vector(), atol=1e-10)
+
+
 def test_simplify_sum_and_ops(S, rng):
     a = ob.random_mps([2] * 6, D=3, complex=True, rng=rng)
     b = ob.random_mps([2] * 6, D=4, complex=True, rng=rng)
