@@ -284,8 +284,8 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     """The same timed sweeps from the same state with the learned-rank split:
     predictor trained on the warm-up's spectra (device kernels), sketched
     randomized SVDs with full-SVD fallback, and the eigensolver's contractions in
-    `precision`.  Returns its rate, the speedup over the plain FP64 path and the
-    energy difference."""
+    `precision`.  Returns the trained predictor and a record with the rate, the
+    speedup over the plain FP64 path and the energy difference."""
     from seemps2_amd import learned
 
     X, t = spectra.tensors()
@@ -321,8 +321,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
         if roof["kernel"] == "svd_jacobi":
             roof["traffic"] = read_traffic("chosen")
-    learned_form.net = net
-    return dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
+    return net, dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
                 bond_updates_at_chi=n_chi, seconds_in_chi_updates=round(t_chi, 4),
                 bond_updates_total=n_all, matvecs_per_update=round(matvecs / max(n_all, 1), 2),
                 speedup_vs_plain=round(value / max(value_plain, 1e-30), 3),
@@ -354,17 +353,21 @@ def simplify_form(QF, H, chi, net, sweeps=2):
 
     def run():
         torch.cuda.synchronize()
+        n0 = learned.SPLITS
         t0 = time.perf_counter()
         out = S.simplify(Hpsi, strategy=st, direction=direction,
                          guess=S.CanonicalMPS(list(psi), center=psi.center, is_canonical=True))
         torch.cuda.synchronize()
-        return out, time.perf_counter() - t0
+        return out, time.perf_counter() - t0, learned.SPLITS - n0
 
     run()  # warm-up (workspace growth)
-    a, t_plain = run()
+    a, t_plain, n_plain = run()
     lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=1, min_size=64)
     with learned.learned_truncation(lt):
-        b, t_learned = run()
+        b, t_learned, n_learned = run()
+    # the stopping rule (error below eps, or growing) may end the two forms
+    # after different numbers of sweeps: compare per two-site update
+    r_plain, r_learned = t_plain / max(n_plain, 1), t_learned / max(n_learned, 1)
 
     def overlap(x, y):
         return complex(S.scprod(x, y))
@@ -374,14 +377,18 @@ def simplify_form(QF, H, chi, net, sweeps=2):
     fa = abs(overlap(a, Hpsi)) / (na * nh)
     fb = abs(overlap(b, Hpsi)) / (nb * nh)
     rel = abs(fa - fb) / fa
-    log(f"simplify at chi={chi} ({sweeps} sweeps of H psi, bond {Hpsi.max_bond_dimension()}): "
-        f"plain {t_plain:.3f} s, learned {t_learned:.3f} s (x{t_plain / t_learned:.2f}), "
+    log(f"simplify at chi={chi} (<= {sweeps} sweeps of H psi, bond {Hpsi.max_bond_dimension()}): "
+        f"plain {t_plain:.3f} s / {n_plain} updates, learned {t_learned:.3f} s / {n_learned} "
+        f"updates (x{r_plain / r_learned:.2f} per update), "
         f"fidelity {fa:.15f} vs {fb:.15f} (rel. diff {rel:.2e}), "
         f"{lt.basis_splits} basis-sketched, {lt.fallbacks} fallbacks of {lt.calls} splits")
-    return dict(workload=f"simplify(H psi) from the chi={chi} DMRG state, {sweeps} sweeps, "
+    return dict(workload=f"simplify(H psi) from the chi={chi} DMRG state, <= {sweeps} sweeps, "
                          f"N={QF.state.size}", seconds_plain=round(t_plain, 4),
-                seconds_learned=round(t_learned, 4),
-                speedup_vs_plain=round(t_plain / t_learned, 3),
+                seconds_learned=round(t_learned, 4), updates_plain=n_plain,
+                updates_learned=n_learned,
+                ms_per_update_plain=round(r_plain * 1e3, 3),
+                ms_per_update_learned=round(r_learned * 1e3, 3),
+                speedup_vs_plain=round(r_plain / r_learned, 3),
                 fidelity_plain=float(fa), fidelity_learned=float(fb),
                 fidelity_rel_diff=float(f"{rel:.3e}"), splits=lt.calls,
                 basis_splits=lt.basis_splits, fallbacks=lt.fallbacks)
@@ -593,13 +600,14 @@ def main():
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
         if roof["kernel"] == "svd_jacobi":
             roof["traffic"] = read_traffic("plain")
-    chosen = None
+    chosen = net = None
     if snapshot is not None and len(spectra.targets) >= 16:
-        chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value, E,
-                              args.sketch_power_iters, args.eig_precision, args.kernel_timing)
+        net, chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value,
+                                   E, args.sketch_power_iters, args.eig_precision,
+                                   args.kernel_timing)
     simp = None
     if chosen is not None and not args.no_simplify:
-        simp = simplify_form(QF, H, chi, learned_form.net)
+        simp = simplify_form(QF, H, chi, net)
     batched = None if args.no_batch else batch_form()
     cpu = None
     if not args.no_cpu_baseline:

@@ -145,8 +145,8 @@ __global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(double2* __restric
 }
 
 // Panel factorisation with the whole panel resident in LDS (nb <= 16 columns
-// of L <= QR_LDS_ROWS rows): one load, four barriers per column (norm |
-// reflector | one wavefront per column dot product | rank-1 update), the T
+// of L <= QR_LDS_ROWS rows, one thread per row): one load, two barriers per
+// column (dot products and norm | reflector and updates, row by row), the T
 // factor from wavefront-parallel dot products, one write-back.  Same outputs
 // as qr_panel_kernel.
 constexpr int QR_LDS_THREADS = 1024;
@@ -159,9 +159,11 @@ __global__ __launch_bounds__(QR_LDS_THREADS) void qr_panel_lds_kernel(
   __shared__ double2 P[QR_LDS_NB * QR_LDS_ROWS];  // column j at P + j * L
   __shared__ double2 s_T[QR_LDS_NB * QR_LDS_NB];
   __shared__ double2 s_y[QR_LDS_NB * QR_LDS_NB];  // y[i][j] = V_i^H V_j, i < j
-  __shared__ double2 s_coef[QR_LDS_NB];
+  __shared__ double2 s_w[QR_LDS_NB];    // unscaled column-j dot products
+  __shared__ double2 s_ccj[QR_LDS_NB];  // trailing columns' row-j entries
   __shared__ double2 s_tau[QR_LDS_NB];
-  __shared__ double s_red[QR_LDS_THREADS / 64];
+  __shared__ double2 s_alpha;
+  __shared__ double s_xn;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nwaves = QR_LDS_THREADS / 64;
   const int L = (int)(len - j0);
@@ -170,16 +172,40 @@ __global__ __launch_bounds__(QR_LDS_THREADS) void qr_panel_lds_kernel(
     P[e] = A[(j0 + j) * len + j0 + rr];
   }
   __syncthreads();
+  // Two barriers per column.  Phase 1 only reads: wavefront q < nt forms the
+  // dot product of the unscaled column j with trailing column j + 1 + q (rows
+  // below j) and keeps that column's row-j entry; wavefront nt forms the norm
+  // of column j below the diagonal and keeps alpha = column j's row j.  Phase 2:
+  // thread rr owns row rr of the panel: it scales row rr of the reflector and
+  // applies every trailing column's update to row rr, so no two threads touch
+  // the same entry and nothing is read after being written.
   for (int j = 0; j < nb; ++j) {
     double2* col = P + j * L;
-    double acc = 0.0;
-    for (int rr = j + 1 + tid; rr < L; rr += QR_LDS_THREADS) acc += cabs2(col[rr]);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) s_red[wave] = acc;
-    const double2 alpha = col[j];
-    __syncthreads();  // #1: partial norms visible, alpha read by everyone
-    double xnorm2 = 0.0;
-    for (int q = 0; q < nwaves; ++q) xnorm2 += s_red[q];
+    const int nt = nb - j - 1;  // trailing columns (<= 15 < nwaves)
+    if (wave < nt) {
+      const double2* cc = P + (j + 1 + wave) * L;
+      double2 a = make_double2(0.0, 0.0);
+      for (int rr = j + 1 + lane; rr < L; rr += 64) a = cfmac(col[rr], cc[rr], a);
+      for (int off = 32; off > 0; off >>= 1) {
+        a.x += __shfl_down(a.x, off, 64);
+        a.y += __shfl_down(a.y, off, 64);
+      }
+      if (lane == 0) {
+        s_w[wave] = a;
+        s_ccj[wave] = cc[j];
+      }
+    } else if (wave == nt) {
+      double acc = 0.0;
+      for (int rr = j + 1 + lane; rr < L; rr += 64) acc += cabs2(col[rr]);
+      for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+      if (lane == 0) {
+        s_xn = acc;
+        s_alpha = col[j];
+      }
+    }
+    __syncthreads();  // #1
+    const double2 alpha = s_alpha;
+    const double xnorm2 = s_xn;
     double2 t = make_double2(0.0, 0.0);
     double beta = alpha.x;
     double2 scal = make_double2(0.0, 0.0);
@@ -191,38 +217,28 @@ __global__ __launch_bounds__(QR_LDS_THREADS) void qr_panel_lds_kernel(
       const double2 d = make_double2(alpha.x - beta, alpha.y);
       const double dd = cabs2(d);
       scal = make_double2(d.x / dd, -d.y / dd);
-      for (int rr = j + 1 + tid; rr < L; rr += QR_LDS_THREADS) col[rr] = cmul(scal, col[rr]);
     }
-    if (tid == 0) {
-      col[j] = make_double2(beta, 0.0);
-      s_tau[j] = t;
-      tau[j0 + j] = t;
-    }
-    __syncthreads();  // #2: v_j complete
-    if (reflect) {
-      // w_jj = v^H P[jj] (v[j] = 1), coef = conj(t) w
-      for (int jj = j + 1 + wave; jj < nb; jj += nwaves) {
-        const double2* cc = P + jj * L;
-        double2 a = make_double2(0.0, 0.0);
-        for (int rr = j + 1 + lane; rr < L; rr += 64) a = cfmac(col[rr], cc[rr], a);
-        for (int off = 32; off > 0; off >>= 1) {
-          a.x += __shfl_down(a.x, off, 64);
-          a.y += __shfl_down(a.y, off, 64);
+    const int rr = tid;
+    if (rr >= j && rr < L) {
+      if (reflect) {
+        // v = [1; scal col[j+1:]];  f_q = conj(t) v^H c_q = conj(t) (c_q[j] + conj(scal) w_q)
+        const double2 ct = make_double2(t.x, -t.y), cs = make_double2(scal.x, -scal.y);
+        const double2 v = (rr == j) ? make_double2(1.0, 0.0) : cmul(scal, col[rr]);
+        for (int q = 0; q < nt; ++q) {
+          const double2 f = cmul(ct, cadd(s_ccj[q], cmul(cs, s_w[q])));
+          double2* cc = P + (j + 1 + q) * L;
+          const double2 p = cmul(f, v);
+          cc[rr] = make_double2(cc[rr].x - p.x, cc[rr].y - p.y);
         }
-        if (lane == 0) s_coef[jj] = cmul(make_double2(t.x, -t.y), cadd(a, cc[j]));
+        if (rr > j) col[rr] = v;
       }
-      __syncthreads();  // #3
-      const int rows = L - j;
-      const int ncols = nb - j - 1;
-      for (int e = tid; e < rows * ncols; e += QR_LDS_THREADS) {
-        const int jj = j + 1 + e / rows, rr = j + e % rows;
-        const double2 v = (rr == j) ? make_double2(1.0, 0.0) : col[rr];
-        const double2 p = cmul(s_coef[jj], v);
-        double2* cc = P + jj * L;
-        cc[rr] = make_double2(cc[rr].x - p.x, cc[rr].y - p.y);
+      if (rr == j) {
+        col[j] = make_double2(beta, 0.0);
+        s_tau[j] = t;
+        tau[j0 + j] = t;
       }
-      __syncthreads();  // #4
     }
+    __syncthreads();  // #2
   }
   // y[i][j] = conj(V_i[j]) + sum_{rr > j} conj(V_i[rr]) V_j[rr], i < j
   for (int q = wave; q < nb * nb; q += nwaves) {

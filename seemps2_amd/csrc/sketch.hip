@@ -327,14 +327,15 @@ __global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __r
     Q[i * GP + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
-  // Diagonal entries below 1e-24 of the largest are noise for the features
-  // (clamped at a ratio of 1e-16 in eigenvalue): they take no rotations, so
-  // the sweeps stop once the resolved part has converged.
+  // Diagonal entries below 1e-16 of the largest are below what the features
+  // resolve (they are clamped at an eigenvalue ratio of 1e-16) and sit on the
+  // Gram's rounding floor: they take no rotations, so the sweeps stop once the
+  // resolved part has converged.
   __shared__ double s_tiny;
   if (tid == 0) {
     double mx = 0.0;
     for (int i = 0; i < PW; ++i) mx = fmax(mx, A[i * GP + i].x);
-    s_tiny = 1e-24 * mx;
+    s_tiny = 1e-16 * mx;
   }
   __syncthreads();
   const double tiny = s_tiny;

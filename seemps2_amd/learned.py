@@ -218,6 +218,8 @@ class LearnedTruncation:
 
 _ACTIVE: LearnedTruncation | None = None
 _RECORDING: SpectrumLog | None = None
+#: Two-site splits performed so far, in every form (for per-update rates).
+SPLITS = 0
 
 
 @contextmanager
@@ -246,6 +248,8 @@ def two_site_split(AA2: torch.Tensor, m: int, n: int, mode: int, strategy, basis
     """The split used by the two-site updates: plain, learned or recorded.
     ``basis`` is the optional canonical neighbour (see LearnedTruncation.split);
     the plain split ignores it."""
+    global SPLITS
+    SPLITS += 1
     if _ACTIVE is not None:
         return _ACTIVE.split(AA2, m, n, mode, strategy, basis)
     A, B, D, err2, _ = K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
