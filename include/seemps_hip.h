@@ -273,8 +273,8 @@ int seemps_two_site_eigensolve_prec(int precision, int64_t chiL, int64_t c, int6
  * Y <- A (A^H Y), Q = orth(Y) (Householder), B = Q^H A, Jacobi SVD of B,
  * U = Q U_B.  The truncation rule runs on B's l singular values with the
  * unsketched weight ||A||_F^2 - ||B||_F^2 counted as discarded.  When that
- * weight exceeds what the rule may discard while max_bond >= l, when the bond
- * cap decides below l but the unsketched weight is neither negligible
+ * weight exceeds what the rule may discard while max_bond > l, when the bond
+ * cap decides at or below l but the unsketched weight is neither negligible
  * (<= 1e-14 ||A||_F^2) nor far below the smallest kept value (<= 1e-3 s_D^2),
  * when a threshold rule would look past l values, for SEEMPS_TRUNC_NONE, or
  * when l >= min(m, n), the full

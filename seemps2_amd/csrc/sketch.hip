@@ -83,8 +83,8 @@ __global__ void truncate_sketch_kernel(int64_t l, int64_t full, const double* __
     const double max_error = total * tol;
     if (tail > max_error) {
       // the tolerance alone would keep more than l values; the sketch still
-      // decides when the bond cap binds below l
-      if (max_bond < l) {
+      // decides when the bond cap binds at or below l (certified as above)
+      if (max_bond <= l) {
         D = max_bond;
         if (!capped_ok(D)) fallback = 1;
       } else {

@@ -192,7 +192,9 @@ class LearnedTruncation:
         l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
         if basis is not None and self.use_basis:
             k = basis.shape[0] if mode == 0 else basis.shape[-1]
-            p = max(self.oversample, l - k)
+            # a bond at the cap cannot grow: the basis alone is the sketch (the
+            # truncation certifies it, or the full SVD decides)
+            p = 0 if k >= strategy.max_bond_dimension else max(self.oversample, l - k)
             if k + p < r:
                 bm = basis.reshape(k, n) if mode == 0 else basis.reshape(m, k)
                 A, B, D, err2, _, fb = K.svd_split_basis(
