@@ -188,22 +188,20 @@ class LearnedTruncation:
             self.full_splits += 1
             return K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
                                strategy.max_bond_dimension)[:4]
-        feats = spectral_features(AA2, m, n, seed=self.seed + self.calls)
-        l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
+        k = None
         if basis is not None and self.use_basis:
             k = basis.shape[0] if mode == 0 else basis.shape[-1]
-            # a bond at the cap cannot grow: the basis alone is the sketch (the
-            # truncation certifies it, or the full SVD decides)
-            p = 0 if k >= strategy.max_bond_dimension else max(self.oversample, l - k)
+            if strategy.max_bond_dimension <= k < r:
+                # a bond at the cap cannot grow: the basis alone is the sketch
+                # (the truncation certifies it, or the full SVD decides), and
+                # no width needs predicting
+                return self._basis_split(AA2, m, n, mode, basis, k, 0, strategy)
+        feats = spectral_features(AA2, m, n, seed=self.seed + self.calls)
+        l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
+        if k is not None:
+            p = max(self.oversample, l - k)
             if k + p < r:
-                bm = basis.reshape(k, n) if mode == 0 else basis.reshape(m, k)
-                A, B, D, err2, _, fb = K.svd_split_basis(
-                    AA2, m, n, mode, bm, k, p, mode, strategy.method, strategy.tolerance,
-                    strategy.max_bond_dimension, seed=self.seed + 7919 * self.calls)
-                self.basis_splits += 1
-                self.sketch_columns += k + p
-                self.fallbacks += int(fb)
-                return A, B, D, err2
+                return self._basis_split(AA2, m, n, mode, basis, k, p, strategy)
         if l >= r:
             self.full_splits += 1
             return K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
@@ -214,6 +212,16 @@ class LearnedTruncation:
                                                   power_iters=self.power_iters,
                                                   seed=self.seed + 7919 * self.calls)
         self.sketch_columns += l
+        self.fallbacks += int(fb)
+        return A, B, D, err2
+
+    def _basis_split(self, AA2, m, n, mode, basis, k, p, strategy):
+        bm = basis.reshape(k, n) if mode == 0 else basis.reshape(m, k)
+        A, B, D, err2, _, fb = K.svd_split_basis(
+            AA2, m, n, mode, bm, k, p, mode, strategy.method, strategy.tolerance,
+            strategy.max_bond_dimension, seed=self.seed + 7919 * self.calls)
+        self.basis_splits += 1
+        self.sketch_columns += k + p
         self.fallbacks += int(fb)
         return A, B, D, err2
 

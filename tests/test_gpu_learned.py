@@ -81,6 +81,42 @@ def test_spectral_features(L, rng):
     assert np.all(f1[1:] < -0.9)
 
 
+@pytest.mark.parametrize("m,n,rate", [(96, 80, 0.35), (64, 200, 0.6), (128, 128, 0.05)])
+def test_spectral_features_against_eigvalsh(rng, m, n, rate):
+    """The features are the sorted log-spectrum of the sketch Gram Y^H Y,
+    Y = A Omega^T: the sketch Omega (32 x n) and the Gram are read back from
+    the caller's workspace (layout: Omega, then Y^T, then G, each padded to
+    256 bytes) and the features are recomputed with numpy's eigvalsh."""
+    from seemps2_amd._lib import check, lib, ptr, stream
+
+    A, _ = decaying(rng, m, n, rate)
+    nbytes = lib.seemps_spectral_sketch_workspace(m, n)
+    ws = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+    f = torch.empty(32, dtype=torch.float32, device="cuda")
+    Ad = torch.tensor(A, device="cuda")
+    check(lib.seemps_spectral_sketch(m, n, ptr(Ad), 11, ptr(f), ptr(ws), nbytes, stream()))
+    torch.cuda.synchronize()
+    al = lambda x: (x + 255) // 256 * 256
+    raw = ws.cpu().numpy()
+    Om = raw[: 32 * n * 16].view(np.complex128).reshape(32, n)
+    g0 = al(32 * n * 16) + al(32 * m * 16)
+    G = raw[g0 : g0 + 32 * 32 * 16].view(np.complex128).reshape(32, 32)
+    Y = A @ Om.T
+    G_ref = Y.conj().T @ Y
+    assert np.linalg.norm(G - G_ref) <= 1e-13 * np.linalg.norm(G_ref)
+    lam = np.sort(np.linalg.eigvalsh(G_ref))[::-1]
+    ratio = np.maximum(lam[1:] / lam[0], 1e-300)
+    f_ref = np.clip(0.5 * np.log10(ratio), -8.0, 0.0) / 8.0
+    f = f.cpu().numpy().astype(np.float64)
+    assert abs(f[0] - np.log10(min(m, n)) / 4) < 1e-6
+    # eigenvalues are known to a few eps times the largest (the Gram's own
+    # rounding): resolved ones match to FP32, the rest sit on that floor
+    resolved = ratio > 1e-9
+    assert resolved.sum() >= 8
+    assert np.abs(f[1:][resolved] - f_ref[resolved]).max() < 2e-5
+    assert np.all(f[1:][~resolved] <= -0.5 * 9 / 8 + 0.02)
+
+
 def _dataset(L, rng, count, tol, size=128):
     feats, targets, mats = [], [], []
     st = ocore.Strategy(tolerance=tol)
