@@ -45,7 +45,7 @@ using jac::inner_jacobi;
 using jac::PW;
 
 constexpr int BW = PW / 2;      // block width (columns); PW, GP from jacobi_inner.h
-constexpr int RT = 32;          // rows per LDS tile
+constexpr int RT = 32;          // rows per LDS tile (the 2 x 2 blocked loops assume 32)
 constexpr int RTP = RT + 1;     // padded tile row length (complex)
 constexpr int RMAX = 16;        // max row chunks per pair
 // Relative off-diagonal measure from which one more sweep is known to suffice
@@ -358,17 +358,45 @@ __device__ void rotate_rows(double2* __restrict__ W, int64_t ldw, int64_t w0, in
       tv[t * RTP + rr] = (rr < nrv) ? V[pair_col(bi, bj, t) * ldv + rbv + rr] : make_double2(0, 0);
     }
     __syncthreads();
-    for (int e = tid; e < PW * RT; e += blockDim.x) {
-      const int j = e / RT, rr = e % RT;
-      double2 aw = make_double2(0.0, 0.0), av = make_double2(0.0, 0.0);
-#pragma unroll 8
+    {
+      // thread (jb, r0): columns jb, jb + 16 by rows r0, r0 + 16 of both tiles
+      const int jb = tid >> 4, r0 = tid & 15;
+      double2 w00 = make_double2(0.0, 0.0), w01 = w00, w10 = w00, w11 = w00;
+      double2 v00 = w00, v01 = w00, v10 = w00, v11 = w00;
+#pragma unroll 4
       for (int k = 0; k < PW; ++k) {
-        const double2 q = Q[k * GP + j];
-        aw = cfma(tw[k * RTP + rr], q, aw);
-        av = cfma(tv[k * RTP + rr], q, av);
+        const double2 q0 = Q[k * GP + jb], q1 = Q[k * GP + jb + 16];
+        const double2 x0 = tw[k * RTP + r0], x1 = tw[k * RTP + r0 + 16];
+        const double2 y0 = tv[k * RTP + r0], y1 = tv[k * RTP + r0 + 16];
+        w00 = cfma(x0, q0, w00);
+        w01 = cfma(x1, q0, w01);
+        w10 = cfma(x0, q1, w10);
+        w11 = cfma(x1, q1, w11);
+        v00 = cfma(y0, q0, v00);
+        v01 = cfma(y1, q0, v01);
+        v10 = cfma(y0, q1, v10);
+        v11 = cfma(y1, q1, v11);
       }
-      if (rr < nrw) W[pair_col(bi, bj, j) * ldw + rbw + rr] = aw;
-      if (rr < nrv) V[pair_col(bi, bj, j) * ldv + rbv + rr] = av;
+      double2* W0 = W + pair_col(bi, bj, jb) * ldw + rbw;
+      double2* W1 = W + pair_col(bi, bj, jb + 16) * ldw + rbw;
+      double2* V0 = V + pair_col(bi, bj, jb) * ldv + rbv;
+      double2* V1 = V + pair_col(bi, bj, jb + 16) * ldv + rbv;
+      if (r0 < nrw) {
+        W0[r0] = w00;
+        W1[r0] = w10;
+      }
+      if (r0 + 16 < nrw) {
+        W0[r0 + 16] = w01;
+        W1[r0 + 16] = w11;
+      }
+      if (r0 < nrv) {
+        V0[r0] = v00;
+        V1[r0] = v10;
+      }
+      if (r0 + 16 < nrv) {
+        V0[r0 + 16] = v01;
+        V1[r0 + 16] = v11;
+      }
     }
     __syncthreads();
   }
@@ -385,12 +413,14 @@ __device__ __forceinline__ int64_t block_owner(int64_t nb, int64_t k, int64_t x)
 }
 
 // Phase A: partial Gram matrix of rows [r0, r1) of the pair's columns into dst
-// (PW x PW, row-major).
+// (PW x PW, row-major).  Thread (ib, jb) accumulates the 2 x 2 block of
+// entries (ib + 16x, jb + 16y): four LDS reads per row feed four complex FMAs.
 __device__ void partial_gram(const SvdShape& s, const SvdWork& w, int64_t bi, int64_t bj,
                              int64_t r0, int64_t r1, double2* __restrict__ dst, double2* tile) {
+  static_assert(PW == 32, "the 2 x 2 register blocking assumes 256 threads and PW = 32");
   const int tid = threadIdx.x;
-  double2 acc[4] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0),
-                    make_double2(0, 0)};
+  const int ib = tid >> 4, jb = tid & 15;
+  double2 a00 = make_double2(0, 0), a01 = a00, a10 = a00, a11 = a00;
   for (int64_t rb = r0; rb < r1; rb += RT) {
     const int nr = (int)min((int64_t)RT, r1 - rb);
     for (int e = tid; e < PW * RT; e += blockDim.x) {
@@ -399,18 +429,23 @@ __device__ void partial_gram(const SvdShape& s, const SvdWork& w, int64_t bi, in
           (rr < nr) ? w.W[pair_col(bi, bj, t) * s.len + rb + rr] : make_double2(0, 0);
     }
     __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int e = tid + u * 256;
-      const int i = e / PW, j = e % PW;
-      double2 a = acc[u];
-      for (int rr = 0; rr < nr; ++rr) a = cfmac(tile[i * RTP + rr], tile[j * RTP + rr], a);
-      acc[u] = a;
+    const double2* x0 = tile + ib * RTP;
+    const double2* x1 = tile + (ib + 16) * RTP;
+    const double2* y0 = tile + jb * RTP;
+    const double2* y1 = tile + (jb + 16) * RTP;
+    for (int rr = 0; rr < nr; ++rr) {
+      const double2 u0 = x0[rr], u1 = x1[rr], v0 = y0[rr], v1 = y1[rr];
+      a00 = cfmac(u0, v0, a00);
+      a01 = cfmac(u0, v1, a01);
+      a10 = cfmac(u1, v0, a10);
+      a11 = cfmac(u1, v1, a11);
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) dst[tid + u * 256] = acc[u];
+  dst[ib * PW + jb] = a00;
+  dst[ib * PW + jb + 16] = a01;
+  dst[(ib + 16) * PW + jb] = a10;
+  dst[(ib + 16) * PW + jb + 16] = a11;
 }
 
 // Phases B + C for item (p, r): G = sum of the R partials at src, the pair's
