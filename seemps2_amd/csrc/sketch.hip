@@ -11,6 +11,7 @@
 // split instead and the fallback flag is set.  Everything stays on the device; the host reads D, the error and the
 // flag once, as seemps_svd_split does.
 #include <cmath>
+#include <initializer_list>
 
 #include "common.h"
 #include "jacobi_inner.h"
@@ -146,8 +147,16 @@ struct SketchWork {
   int* fdev;
   void* svdw;
   int64_t svd_bytes;
+  double2* sk;  // split-K partial products (zgemm_ws) of the few-tile products
+  int64_t sk_elems;
   int64_t bytes;
 };
+
+inline int64_t max_of(std::initializer_list<int64_t> xs) {
+  int64_t r = 0;
+  for (int64_t x : xs) r = x > r ? x : r;
+  return r;
+}
 
 // base == nullptr only sizes the layout (w.bytes).
 SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
@@ -182,6 +191,9 @@ SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
   w.fdev = (int*)take(8);
   w.svd_bytes = svd_workspace_bytes(l, n);
   w.svdw = take(w.svd_bytes);
+  w.sk_elems = max_of({splitk_workspace(l, m, n), splitk_workspace(l, n, m),
+                       splitk_workspace(m, n, l), splitk_workspace(m, l, l)});
+  w.sk = (double2*)take(w.sk_elems * 16);
   w.bytes = off;
   return w;
 }
@@ -263,6 +275,8 @@ struct BasisWork {
   int* fdev;
   void* svdw;
   int64_t svd_bytes;
+  double2* sk;  // split-K partial products (zgemm_ws) of the few-tile products
+  int64_t sk_elems;
   int64_t bytes;
 };
 
@@ -300,6 +314,12 @@ BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) 
   w.fdev = (int*)take(8);
   w.svd_bytes = svd_workspace_bytes(Lg, l, false);
   w.svdw = take(w.svd_bytes);
+  // the factor products (D <= l columns) split when D = l fits
+  w.sk_elems = max_of({splitk_workspace(pp, Lg, Sh), splitk_workspace(pp, Sh, Lg),
+                       splitk_workspace(pp, k, Sh), splitk_workspace(pp, Sh, k),
+                       splitk_workspace(Lg, l, Sh), splitk_workspace(Lg, Sh, l),
+                       splitk_workspace(l, Sh, l), splitk_workspace(Sh, l, l)});
+  w.sk = (double2*)take(w.sk_elems * 16);
   w.bytes = off;
   return w;
 }
@@ -371,9 +391,19 @@ __global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __r
 }  // namespace
 }  // namespace seemps
 
+namespace seemps {
+namespace {
+// split-K scratch of the spectral sketch's two products (after Omega, Y^T and G)
+int64_t spectral_sk_elems(int64_t m, int64_t n) {
+  return max_of({splitk_workspace(NF, m, n), splitk_workspace(NF, NF, m)});
+}
+}  // namespace
+}  // namespace seemps
+
 extern "C" int64_t seemps_spectral_sketch_workspace(int64_t m, int64_t n) {
+  using namespace seemps;
   auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
-  return al(seemps::NF * n * 16) + al(seemps::NF * m * 16) + al(seemps::NF * seemps::NF * 16);
+  return al(NF * n * 16) + al(NF * m * 16) + al(NF * NF * 16) + spectral_sk_elems(m, n) * 16;
 }
 
 extern "C" int seemps_spectral_sketch(int64_t m, int64_t n, const void* A, uint64_t seed,
@@ -391,16 +421,19 @@ extern "C" int seemps_spectral_sketch(int64_t m, int64_t n, const void* A, uint6
   double2* Yt = (double2*)p;
   p += al(NF * m * 16);
   double2* G = (double2*)p;
+  p += al(NF * NF * 16);
+  double2* sk = (double2*)p;
+  const int64_t sk_elems = spectral_sk_elems(m, n);
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   int st;
   hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(NF * n)), dim3(256), 0, st_, NF * n, seed, Om);
   if ((st = check_launch("gauss"))) return st;
   // Y^T = Omega^T A^T (32 x m); G = conj(Y^T) (Y^T)^T = Y^H Y
-  if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, NF, m, n, one, Om, n, 0, A, n, 0, zero, Yt, m, 0, 1,
-                  st_)))
+  if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, NF, m, n, one, Om, n, A, n, zero, Yt, m, sk,
+                     sk_elems, st_)))
     return st;
-  if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_T, NF, NF, m, one, Yt, m, 0, Yt, m, 0, zero, G, NF, 0, 1,
-                  st_)))
+  if ((st = zgemm_ws(SEEMPS_OP_C, SEEMPS_OP_T, NF, NF, m, one, Yt, m, Yt, m, zero, G, NF, sk,
+                     sk_elems, st_)))
     return st;
   hipLaunchKernelGGL(sketch_features_kernel, dim3(1), dim3(256), 0, st_, G, m < n ? m : n,
                      features);
@@ -446,25 +479,25 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
     // Y^T = Omega^T A^T, stored column-major (l columns of length m)
     hipLaunchKernelGGL(gauss_kernel, dim3(grid_for(l * n)), dim3(256), 0, st_, l * n, seed, w.Om);
     if ((st = check_launch("gauss"))) return st;
-    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.Om, n, 0, A, n, 0, zero, w.Ycm, m, 0, 1,
-                    st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.Om, n, A, n, zero, w.Ycm, m, w.sk,
+                       w.sk_elems, st_)))
       return st;
     for (int64_t it = 0; it < power_iters; ++it) {
       if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
       // Z^T = Q^T conj(A): the columns of A^H Q
-      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.Zcm, n,
-                      0, 1, st_)))
+      if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_C, l, n, m, one, w.Qcm, m, A, n, zero, w.Zcm, n,
+                         w.sk, w.sk_elems, st_)))
         return st;
       if ((st = orthonormalize(w.Zcm, n, l, w.B, w, st_))) return st;  // B holds Qz here
       // Y^T = Qz^T A^T
-      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.B, n, 0, A, n, 0, zero, w.Ycm, m, 0,
-                      1, st_)))
+      if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, l, m, n, one, w.B, n, A, n, zero, w.Ycm, m, w.sk,
+                         w.sk_elems, st_)))
         return st;
     }
     if ((st = orthonormalize(w.Ycm, m, l, w.Qcm, w, st_))) return st;
     // B = Q^H A (l x n)
-    if ((st = zgemm(SEEMPS_OP_C, SEEMPS_OP_N, l, n, m, one, w.Qcm, m, 0, A, n, 0, zero, w.B, n, 0, 1,
-                    st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_C, SEEMPS_OP_N, l, n, m, one, w.Qcm, m, A, n, zero, w.B, n, w.sk,
+                       w.sk_elems, st_)))
       return st;
   }
   int off = 0;
@@ -477,8 +510,8 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
   if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
   // R = A - Q B (m x n), Q^T stored l x m
   SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
-  if ((st = zgemm(SEEMPS_OP_T, SEEMPS_OP_N, m, n, l, make_double2(-1.0, 0.0), w.Qcm, m, 0, w.B, n,
-                  0, one, w.Rres, n, 0, 1, st_)))
+  if ((st = zgemm_ws(SEEMPS_OP_T, SEEMPS_OP_N, m, n, l, make_double2(-1.0, 0.0), w.Qcm, m, w.B, n,
+                     one, w.Rres, n, w.sk, w.sk_elems, st_)))
     return st;
   if ((st = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return st;
   hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
@@ -495,8 +528,8 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
   if (hfb) return full_split();
   const int64_t D = hD;
   // U = Q U_B (m x l): (Q^T)^T U_B with Q^T stored l x m
-  if ((st = zgemm(SEEMPS_OP_T, SEEMPS_OP_N, m, D, l, one, w.Qcm, m, 0, w.Ub, l, 0, zero, w.U, l, 0,
-                  1, st_)))
+  if ((st = zgemm_ws(SEEMPS_OP_T, SEEMPS_OP_N, m, D, l, one, w.Qcm, m, w.Ub, l, zero, w.U, l, w.sk,
+                     w.sk_elems, st_)))
     return st;
   if (mode == 0) {
     if ((st = zscale2d_impl(m, D, nullptr, nullptr, w.U, l, (double2*)outA, D, st_))) return st;
@@ -559,19 +592,19 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
       if ((st = check_launch("gauss"))) return st;
       if (side == 0) {
         // Yt = G0^T A^T (p x m); Zt = Yt conj(A) (p x n)
-        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.G0t, n, 0, A, n, 0, zero, w.Yt,
-                        m, 0, 1, st_)))
+        if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.G0t, n, A, n, zero, w.Yt, m,
+                           w.sk, w.sk_elems, st_)))
           return st;
-        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.Yt, m, 0, A, n, 0, zero, w.Zt,
-                        n, 0, 1, st_)))
+        if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.Yt, m, A, n, zero, w.Zt, n,
+                           w.sk, w.sk_elems, st_)))
           return st;
       } else {
         // Yt = G0^T conj(A) (p x n); Zt = Yt A^T (p x m)
-        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.G0t, m, 0, A, n, 0, zero, w.Yt,
-                        n, 0, 1, st_)))
+        if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_C, p, n, m, one, w.G0t, m, A, n, zero, w.Yt, n,
+                           w.sk, w.sk_elems, st_)))
           return st;
-        if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.Yt, n, 0, A, n, 0, zero, w.Zt,
-                        m, 0, 1, st_)))
+        if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, p, m, n, one, w.Yt, n, A, n, zero, w.Zt, m,
+                           w.sk, w.sk_elems, st_)))
           return st;
       }
       // X -= Q (Q^H X) for the p columns of Xt (column-major, length Sh);
@@ -579,18 +612,18 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
       auto project = [&](double2* Xt) -> int {
         int e;
         if (side == 0) {  // Q^H = basis (k x n): conj(Q) = basis^T, Q^T = conj(basis)
-          if ((e = zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, k, n, one, Xt, n, 0, Bs, n, 0, zero, w.C, k,
-                         0, 1, st_)))
+          if ((e = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, p, k, n, one, Xt, n, Bs, n, zero, w.C, k,
+                            w.sk, w.sk_elems, st_)))
             return e;
-          return zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, n, k, mone, w.C, k, 0, Bs, n, 0, one, Xt, n,
-                       0, 1, st_);
+          return zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_C, p, n, k, mone, w.C, k, Bs, n, one, Xt, n, w.sk,
+                          w.sk_elems, st_);
         }
         // Q = basis (m x k): conj(Q) = conj(basis), Q^T = basis^T
-        if ((e = zgemm(SEEMPS_OP_N, SEEMPS_OP_C, p, k, m, one, Xt, m, 0, Bs, k, 0, zero, w.C, k,
-                       0, 1, st_)))
+        if ((e = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_C, p, k, m, one, Xt, m, Bs, k, zero, w.C, k, w.sk,
+                          w.sk_elems, st_)))
           return e;
-        return zgemm(SEEMPS_OP_N, SEEMPS_OP_T, p, m, k, mone, w.C, k, 0, Bs, k, 0, one, Xt, m, 0,
-                     1, st_);
+        return zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, p, m, k, mone, w.C, k, Bs, k, one, Xt, m, w.sk,
+                        w.sk_elems, st_);
       };
       // Project twice, orthonormalise; then once more of each: when A lies (almost)
       // in the basis span, Z is rank-deficient and the Householder factor's extra
@@ -606,12 +639,12 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
     if ((st = check_launch("build_omega_h"))) return st;
     // W = X Omega = X Oh^H (Lg x l)
     if (side == 0) {
-      if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_H, m, l, n, one, A, n, 0, w.Oh, n, 0, zero, w.W, l,
-                      0, 1, st_)))
+      if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_H, m, l, n, one, A, n, w.Oh, n, zero, w.W, l, w.sk,
+                         w.sk_elems, st_)))
         return st;
     } else {
-      if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, n, l, m, one, A, n, 0, w.Oh, m, 0, zero, w.W, l,
-                      0, 1, st_)))
+      if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_H, n, l, m, one, A, n, w.Oh, m, zero, w.W, l, w.sk,
+                         w.sk_elems, st_)))
         return st;
     }
   }
@@ -626,12 +659,12 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
   // R = A - (projection of A on Omega): side 0: A - W Oh; side 1: A - Oh^H W^H
   SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
   if (side == 0) {
-    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, m, n, l, mone, w.W, l, 0, w.Oh, n, 0, one, w.Rres,
-                    n, 0, 1, st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, m, n, l, mone, w.W, l, w.Oh, n, one, w.Rres, n,
+                       w.sk, w.sk_elems, st_)))
       return st;
   } else {
-    if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, m, n, l, mone, w.Oh, m, 0, w.W, l, 0, one, w.Rres,
-                    n, 0, 1, st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_H, m, n, l, mone, w.Oh, m, w.W, l, one, w.Rres, n,
+                       w.sk, w.sk_elems, st_)))
       return st;
   }
   if ((st = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return st;
@@ -652,16 +685,16 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
   const double* sB = (mode == 0) ? w.S : nullptr;  // S into the right factor
   if (side == 0) {
     // A = Uw S (VHw Oh): left = Uw[:, :D], right = VHw[:D] Oh (D x n)
-    if ((st = zgemm(SEEMPS_OP_N, SEEMPS_OP_N, D, n, l, one, w.VHw, l, 0, w.Oh, n, 0, zero, w.T, n,
-                    0, 1, st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, D, n, l, one, w.VHw, l, w.Oh, n, zero, w.T, n,
+                       w.sk, w.sk_elems, st_)))
       return st;
     if ((st = zscale2d_impl(m, D, nullptr, sA, w.Uw, l, (double2*)outA, D, st_))) return st;
     if ((st = zscale2d_impl(D, n, sB, nullptr, w.T, n, (double2*)outB, n, st_))) return st;
   } else {
     // A^H = Uw S (VHw Oh): A = (Oh^H VHw^H) S Uw^H; left = Oh^H VHw[:D]^H (m x D),
     // right = Uw[:, :D]^H (D x n)
-    if ((st = zgemm(SEEMPS_OP_H, SEEMPS_OP_H, m, D, l, one, w.Oh, m, 0, w.VHw, l, 0, zero, w.T, D,
-                    0, 1, st_)))
+    if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_H, m, D, l, one, w.Oh, m, w.VHw, l, zero, w.T, D,
+                       w.sk, w.sk_elems, st_)))
       return st;
     if ((st = zscale2d_impl(m, D, nullptr, sA, w.T, D, (double2*)outA, D, st_))) return st;
     hipLaunchKernelGGL(adjoint_scale, dim3(grid_for(D * n)), dim3(256), 0, st_, D, n, w.Uw, l, sB,
