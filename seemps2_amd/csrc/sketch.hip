@@ -327,64 +327,153 @@ BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) 
 constexpr int NF = 32;  // sketch columns = features of the rank predictor
 
 // Features of the rank predictor from the 32 x 32 Gram matrix G = Y^H Y of the
-// sketch Y = A Omega: eigenvalues by cyclic Jacobi sweeps (one workgroup, the
-// block-Jacobi inner solver), sorted; f[0] = log10(min(m, n)) / 4,
-// f[i] = clamp(log10(sigma_i / sigma_0), -8, 0) / 8 for i = 1..31 (the Gram
-// of the sketch resolves ratios down to ~sqrt(eps), hence the clamp).
+// sketch Y = A Omega, one workgroup of 256 threads: G (scaled by its largest
+// diagonal entry) is reduced to a real symmetric tridiagonal by Householder
+// reflections, and each eigenvalue is found by 18 rounds of 9-section on
+// Sturm counts (8 lanes per eigenvalue, one shift each; the interval shrinks
+// 9^18 ~ 1.5e17 times from the Gershgorin bounds).  Sorted descending:
+// f[0] = log10(min(m, n)) / 4, f[i] = clamp(log10(sigma_i / sigma_0), -8, 0) / 8
+// for i = 1..31 (the Gram of the sketch resolves ratios down to ~sqrt(eps),
+// hence the clamp).
 __global__ __launch_bounds__(256) void sketch_features_kernel(const double2* __restrict__ G,
                                                               int64_t r, float* __restrict__ f) {
-  using namespace jac;
-  __shared__ double2 A[PW * GP];
-  __shared__ double2 A2[PW * GP];
-  __shared__ double2 Q[PW * GP];
-  __shared__ double2 rot[PW];
-  __shared__ int active[2];
-  __shared__ double lam[PW];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < PW * PW; e += blockDim.x) {
-    const int i = e / PW, j = e % PW;
-    A[i * GP + j] = G[e];
-    Q[i * GP + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-  }
-  __syncthreads();
-  // Diagonal entries below 1e-16 of the largest are below what the features
-  // resolve (they are clamped at an eigenvalue ratio of 1e-16) and sit on the
-  // Gram's rounding floor: they take no rotations, so the sweeps stop once the
-  // resolved part has converged.
-  __shared__ double s_tiny;
+  using jac::GP;
+  constexpr int n = NF;
+  constexpr double PIVMIN = 1e-290;  // smallest pivot magnitude of a Sturm count
+  __shared__ double2 A[n * GP];
+  __shared__ double2 v[n], pw[n];   // reflector, then p = beta A v and w = p - K v
+  __shared__ double d[n], e2[n];    // tridiagonal: diagonal, squared off-diagonal
+  __shared__ double lam[n];         // eigenvalues, ascending
+  __shared__ double s_scale, s_beta;
+  __shared__ int s_skip[2];
+  const int tid = threadIdx.x, lane = tid & 63;
   if (tid == 0) {
     double mx = 0.0;
-    for (int i = 0; i < PW; ++i) mx = fmax(mx, A[i * GP + i].x);
-    s_tiny = 1e-16 * mx;
+    for (int i = 0; i < n; ++i) mx = fmax(mx, G[i * n + i].x);
+    s_scale = mx;
   }
   __syncthreads();
-  const double tiny = s_tiny;
-  double2* cur = A;
-  for (int sw = 0; sw < 12; ++sw) {
-    int nact = 0;
-    cur = inner_jacobi(cur, cur == A ? A2 : A, Q, rot, active, 1e-12, tiny, 1, &nact);
-    __syncthreads();  // the next call rewrites the round flags
-    if (nact == 0) break;  // a sweep without rotations: converged (uniform)
+  const double scale = s_scale;
+  const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const double2 g = G[e];
+    A[(e / n) * GP + e % n] = make_double2(g.x * inv, g.y * inv);
   }
-  if (tid < PW) lam[tid] = cur[tid * GP + tid].x;
   __syncthreads();
-  if (tid == 0) {
-    for (int i = 1; i < PW; ++i) {  // insertion sort, descending
-      const double x = lam[i];
-      int j = i - 1;
-      while (j >= 0 && lam[j] < x) {
-        lam[j + 1] = lam[j];
-        --j;
+  for (int j = 0; j < n - 2; ++j) {
+    const int L = n - 1 - j;  // trailing rows / columns j + 1 .. n - 1
+    if (tid < 64) {
+      // reflector of column j below the diagonal: v = x - alpha e_0,
+      // alpha = -phase(x_0) |x|, so that H x = alpha e_0 with H = I - beta v v^H
+      const double2 x = (lane < L) ? A[(j + 1 + lane) * GP + j] : make_double2(0.0, 0.0);
+      double tail2 = (lane >= 1) ? cabs2(x) : 0.0;
+      for (int o = 32; o > 0; o >>= 1) tail2 += __shfl_xor(tail2, o, 64);
+      const double2 x0 = make_double2(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
+      const double a02 = cabs2(x0), a0 = sqrt(a02);
+      const int skip = !(tail2 > 1e-200);  // column already reduced (or below any resolution)
+      if (lane == 0) {
+        d[j] = A[j * GP + j].x;
+        e2[j] = skip ? a02 : a02 + tail2;
+        s_skip[j & 1] = skip;
       }
-      lam[j + 1] = x;
+      if (!skip) {
+        const double xn = sqrt(a02 + tail2);
+        const double2 ph = a0 > 0.0 ? make_double2(x0.x / a0, x0.y / a0) : make_double2(1.0, 0.0);
+        if (lane < L) v[lane] = (lane == 0) ? make_double2(ph.x * (a0 + xn), ph.y * (a0 + xn)) : x;
+        if (lane == 0) s_beta = 2.0 / ((a0 + xn) * (a0 + xn) + tail2);
+      }
     }
-    f[0] = (float)(log10((double)r) / 4.0);
-    const double l0 = lam[0];
-    for (int i = 1; i < NF; ++i) {
-      double v = (l0 > 0.0 && lam[i] > 0.0) ? 0.5 * log10(lam[i] / l0) : -8.0;
-      v = v < -8.0 ? -8.0 : (v > 0.0 ? 0.0 : v);
-      f[i] = (float)(v / 8.0);
+    __syncthreads();
+    if (s_skip[j & 1]) continue;  // the flag is double-buffered: wave 0 may run ahead
+    const double beta = s_beta;
+    {
+      // p = beta A v: 8 lanes per row
+      const int t = tid >> 3, sub = tid & 7;
+      double2 acc = make_double2(0.0, 0.0);
+      if (t < L)
+        for (int k = sub; k < L; k += 8) acc = cfma(A[(j + 1 + t) * GP + j + 1 + k], v[k], acc);
+      for (int o = 1; o < 8; o <<= 1) {
+        acc.x += __shfl_xor(acc.x, o, 64);
+        acc.y += __shfl_xor(acc.y, o, 64);
+      }
+      if (t < L && sub == 0) pw[t] = make_double2(beta * acc.x, beta * acc.y);
     }
+    __syncthreads();
+    if (tid < 64) {
+      // w = p - K v, K = (beta / 2) v^H p (real: A is Hermitian)
+      const double2 vv = lane < L ? v[lane] : make_double2(0.0, 0.0);
+      const double2 pp = lane < L ? pw[lane] : make_double2(0.0, 0.0);
+      double dot = vv.x * pp.x + vv.y * pp.y;
+      for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
+      const double K = 0.5 * beta * dot;
+      if (lane < L) pw[lane] = make_double2(pp.x - K * vv.x, pp.y - K * vv.y);
+    }
+    __syncthreads();
+    // trailing block: A <- A - v w^H - w v^H
+    for (int e = tid; e < L * L; e += blockDim.x) {
+      const int rr = e / L, cc = e % L;
+      const double2 vr = v[rr], wr = pw[rr], vc = v[cc], wc = pw[cc];
+      double2 a = A[(j + 1 + rr) * GP + j + 1 + cc];
+      a.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
+      a.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
+      A[(j + 1 + rr) * GP + j + 1 + cc] = a;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    d[n - 2] = A[(n - 2) * GP + n - 2].x;
+    e2[n - 2] = cabs2(A[(n - 1) * GP + n - 2]);
+    d[n - 1] = A[(n - 1) * GP + n - 1].x;
+  }
+  __syncthreads();
+  double dd[n], ee[n - 1];
+  double glo = 1e300, ghi = -1e300;
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    dd[i] = d[i];
+    if (i < n - 1) ee[i] = e2[i];
+  }
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    const double rad = (i > 0 ? sqrt(ee[i - 1]) : 0.0) + (i < n - 1 ? sqrt(ee[i]) : 0.0);
+    glo = fmin(glo, dd[i] - rad);
+    ghi = fmax(ghi, dd[i] + rad);
+  }
+  const double span = ghi - glo;
+  double lo = glo - 1e-14 * span - 1e-300, hi = ghi + 1e-14 * span + 1e-300;
+  // eigenvalue i (ascending) by the 8-lane group of this thread; lane g of the
+  // group counts the eigenvalues below lo + (g + 1) h, h = (hi - lo) / 9
+  const int i = tid >> 3, g = tid & 7;
+  for (int it = 0; it < 18; ++it) {
+    const double h = (hi - lo) * (1.0 / 9.0);
+    const double x = lo + (double)(g + 1) * h;
+    double q = dd[0] - x;
+    if (fabs(q) < PIVMIN) q = -PIVMIN;
+    int cnt = q < 0.0;
+#pragma unroll
+    for (int k = 1; k < n; ++k) {
+      q = (dd[k] - x) - ee[k - 1] * jac::rcp_nr(q);
+      if (fabs(q) < PIVMIN) q = -PIVMIN;
+      cnt += q < 0.0;
+    }
+    // first shift of the group with more than i eigenvalues below it
+    const unsigned long long m = __ballot(cnt > i);
+    const unsigned bits = (unsigned)(m >> ((lane >> 3) * 8)) & 0xFFu;
+    const int jj = bits ? __builtin_ctz(bits) : 8;
+    const double nlo = lo + (double)jj * h;
+    const double nhi = (jj == 8) ? hi : lo + (double)(jj + 1) * h;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (g == 0) lam[i] = 0.5 * (lo + hi) * scale;
+  __syncthreads();
+  if (tid == 0) f[0] = (float)(log10((double)r) / 4.0);
+  if (tid >= 1 && tid < n) {
+    // descending order: sigma_k^2 = lam[n - 1 - k]
+    const double l0 = lam[n - 1], li = lam[n - 1 - tid];
+    double val = (l0 > 0.0 && li > 0.0) ? 0.5 * log10(li / l0) : -8.0;
+    val = val < -8.0 ? -8.0 : (val > 0.0 ? 0.0 : val);
+    f[tid] = (float)(val / 8.0);
   }
 }
 
