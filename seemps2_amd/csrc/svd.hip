@@ -296,7 +296,7 @@ __global__ void load_y(SvdShape s, SvdWork w) {
 __device__ bool grid_barrier(Ctrl* ctrl, unsigned int target, int* s_flag) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish, then arrive
     atomicAdd(&ctrl->barrier, 1u);
     unsigned long long spins = 0;
     int ok = 1;
@@ -313,7 +313,7 @@ __device__ bool grid_barrier(Ctrl* ctrl, unsigned int target, int* s_flag) {
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see every arrival's stores
     *s_flag = ok;
   }
   __syncthreads();
@@ -548,7 +548,7 @@ __device__ bool wait_counters(Ctrl* ctrl, const unsigned int* c0, const unsigned
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the signallers' stores
     *s_flag = ok;
   }
   __syncthreads();
@@ -556,18 +556,21 @@ __device__ bool wait_counters(Ctrl* ctrl, const unsigned int* c0, const unsigned
 }
 
 // Publish this block's writes, then add 1 to / set a counter.
+// Signals publish this workgroup's stores: the barrier gathers them in thread
+// 0, whose release fence precedes the counter update; a wait that observes the
+// counter acquires them (and its barrier passes them on to its workgroup).
 __device__ void signal_add(unsigned int* c) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     atomicAdd(c, 1u);
   }
 }
 __device__ void signal_set(unsigned int* c, unsigned int v) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(c, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
