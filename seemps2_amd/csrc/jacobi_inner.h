@@ -78,6 +78,14 @@ __device__ __forceinline__ void inner_pair(int k, int t, int* a, int* b) {
   }
 }
 
+// Index pair t of round k (0 <= k < PW/2) of the cross-half schedule: index t
+// of the first half against index PW/2 + (t + k) mod PW/2 of the second, so the
+// PW/2 rounds visit every pair across the halves once and none within a half.
+__device__ __forceinline__ void cross_pair(int k, int t, int* a, int* b) {
+  *a = t;
+  *b = PW / 2 + ((t + k) & (PW / 2 - 1));
+}
+
 // Inner cyclic Jacobi sweep(s) on the Hermitian PW x PW matrix G (LDS, row
 // stride GP; G2 is a second buffer of the same shape), accumulating Q (LDS,
 // on entry identity).  256 threads.  Each round: the first 16 lanes derive the
@@ -87,22 +95,30 @@ __device__ __forceinline__ void inner_pair(int k, int t, int* a, int* b) {
 // two rows of Q for pair p; a round whose 16 rotations are all the identity
 // skips that phase.  Returns the buffer holding the final G.  When nactive is
 // given, *nactive (thread-private, the same value in every thread) receives
-// the number of rounds that rotated anything.
+// the number of rounds that rotated anything.  cross: each sweep is the PW/2
+// rounds of the cross-half schedule instead of the PW - 1 rounds of the circle.
 __device__ inline double2* inner_jacobi(double2* G, double2* G2, double2* Q, double2* rot, int* active,
                                  double tol, double tiny2, int sweeps,
-                                 int* nactive = nullptr) {
+                                 int* nactive = nullptr, bool cross = false) {
   const int tid = threadIdx.x;
   const int p = tid >> 4, q = tid & 15;
   double2* cur = G;
   double2* nxt = G2;
   int rnd = 0;
+  const int rounds = cross ? PW / 2 : PW - 1;
+  auto pair_of = [cross](int k, int t, int* a, int* b) {
+    if (cross)
+      cross_pair(k, t, a, b);
+    else
+      inner_pair(k, t, a, b);
+  };
   for (int sw = 0; sw < sweeps; ++sw) {
-    for (int k = 0; k < PW - 1; ++k) {
+    for (int k = 0; k < rounds; ++k) {
       if (tid < 64) {
         int any = 0;
         if (tid < PW / 2) {
           int a, b;
-          inner_pair(k, tid, &a, &b);
+          pair_of(k, tid, &a, &b);
           double cs, sn;
           double2 ph;
           jacobi_rotation(cur, a, b, tol, tiny2, &cs, &sn, &ph);
@@ -119,8 +135,8 @@ __device__ inline double2* inner_jacobi(double2* G, double2* G2, double2* Q, dou
       if (!active[(rnd++) & 1]) continue;
       if (nactive) ++*nactive;
       int a, b, c, d;
-      inner_pair(k, p, &a, &b);
-      inner_pair(k, q, &c, &d);
+      pair_of(k, p, &a, &b);
+      pair_of(k, q, &c, &d);
       const double2 rp = rot[2 * p], php = rot[2 * p + 1];
       const double2 rq = rot[2 * q], phq = rot[2 * q + 1];
       const double csp = rp.x, snp = rp.y, csq = rq.x, snq = rq.y;
