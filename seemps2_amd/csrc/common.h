@@ -80,6 +80,15 @@ __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Pinned host buffer of at least `bytes` for small device-to-host reads (one
+// per host thread; nullptr if it cannot be allocated).  The caller copies into
+// it on its stream and synchronises before reading.
+void* pinned_stage(size_t bytes);
+// Layout of the staging buffer: scalars at [0, PINNED_CTRL_OFF), the SVD
+// control block at [PINNED_CTRL_OFF, PINNED_BYTES).
+constexpr size_t PINNED_CTRL_OFF = 1024;
+constexpr size_t PINNED_BYTES = 4096;
+
 // Device properties cached per device (CU count, cooperative limits).
 struct DeviceInfo {
   int device;

@@ -6,6 +6,7 @@
 // 187-237) and the scipy eigsh / ARPACK call of _diagonalize_two_site
 // (src/seemps/optimization/dmrg.py:15-23).
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
 #include "common.h"
@@ -656,6 +657,10 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
   hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, w.V);
   if ((st = check_launch("lz_unit"))) return st;
   double theta = 0.0, resid = 0.0;
+  double* hp = (double*)pinned_stage(PINNED_BYTES);  // host reads of the scalars
+  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
+  static_assert(offsetof(LzScalars, resid) == offsetof(LzScalars, theta) + 8,
+                "theta and resid are read together");
   int64_t matvecs = 0, matvecs32 = 0, cycle = 0;
   bool converged = false;
   // FP32 phase: the operator is applied with FP32 products, whose rounding
@@ -714,7 +719,7 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
       }
       if ((st = check_launch("lanczos step"))) return st;
       if (prec == SEEMPS_PREC_MIXED && cycle == 0 && j == 0) {
-        double ab[2];
+        double* ab = hp;
         SEEMPS_HIP(hipMemcpyAsync(&ab[0], &w.sc->alpha[0], 8, hipMemcpyDeviceToHost, st_));
         SEEMPS_HIP(hipMemcpyAsync(&ab[1], &w.sc->beta[0], 8, hipMemcpyDeviceToHost, st_));
         SEEMPS_HIP(hipStreamSynchronize(st_));
@@ -725,9 +730,8 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
           (kd - first_check) % check_every == 0) {
         hipLaunchKernelGGL(lz_tridiag, dim3(1), dim3(64), 0, st_, w.sc, (int)kd);
         if ((st = check_launch("lz_tridiag"))) return st;
-        double h[2];
-        SEEMPS_HIP(hipMemcpyAsync(&h[0], &w.sc->theta, 8, hipMemcpyDeviceToHost, st_));
-        SEEMPS_HIP(hipMemcpyAsync(&h[1], &w.sc->resid, 8, hipMemcpyDeviceToHost, st_));
+        double* h = hp;  // theta and resid are adjacent in LzScalars
+        SEEMPS_HIP(hipMemcpyAsync(h, &w.sc->theta, 16, hipMemcpyDeviceToHost, st_));
         SEEMPS_HIP(hipStreamSynchronize(st_));
         if (h[1] <= tol * fmax(eps23, fabs(h[0]))) {
           kd_eff = kd;
@@ -748,9 +752,8 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
     if ((st = zmdotc_impl(n, 1, x, n, x, &w.sc->nrm, w.part, st_))) return st;
     hipLaunchKernelGGL(lz_unit, dim3(gb), dim3(256), 0, st_, n, w.sc, x);
     SEEMPS_HIP(hipMemcpyAsync(w.V, x, n * 16, hipMemcpyDeviceToDevice, st_));
-    double h[2];
-    SEEMPS_HIP(hipMemcpyAsync(&h[0], &w.sc->theta, 8, hipMemcpyDeviceToHost, st_));
-    SEEMPS_HIP(hipMemcpyAsync(&h[1], &w.sc->resid, 8, hipMemcpyDeviceToHost, st_));
+    double* h = hp;
+    SEEMPS_HIP(hipMemcpyAsync(h, &w.sc->theta, 16, hipMemcpyDeviceToHost, st_));
     SEEMPS_HIP(hipStreamSynchronize(st_));
     theta = h[0];
     resid = h[1];

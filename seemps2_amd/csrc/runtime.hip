@@ -28,6 +28,25 @@ int check_launch(const char* what) {
   return SEEMPS_OK;
 }
 
+// Pinned host staging for the small device-to-host reads of the hot paths
+// (eigensolver exits, split ranks, SVD control blocks): copies into pinned
+// memory are plain DMA, where copies into pageable memory go through a
+// runtime bounce buffer.  One buffer per host thread, allocated on first use
+// and kept for the process; a request larger than the buffer replaces it.
+void* pinned_stage(size_t bytes) {
+  static thread_local void* buf = nullptr;
+  static thread_local size_t cap = 0;
+  if (bytes > cap) {
+    const size_t want = std::max<size_t>(bytes, 4096);
+    void* p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+    if (buf) (void)hipHostFree(buf);
+    buf = p;
+    cap = want;
+  }
+  return buf;
+}
+
 int device_info(DeviceInfo* info) {
   // One cache entry per device; the library is used from one process per GPU.
   static thread_local int cached_dev = -1;

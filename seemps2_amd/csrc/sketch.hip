@@ -186,9 +186,13 @@ SketchWork sketch_carve(int64_t m, int64_t n, int64_t l, void* base) {
   w.Rres = (double2*)take(m * n * 16);
   w.resid2 = (double2*)take(16);
   w.S = (double*)take(l * 8);
-  w.Ddev = (int64_t*)take(8);
-  w.edev = (double*)take(8);
-  w.fdev = (int*)take(8);
+  {
+    // D, discarded weight and fallback flag side by side: one read to the host
+    char* rk = (char*)take(32);
+    w.Ddev = (int64_t*)rk;
+    w.edev = rk ? (double*)(rk + 8) : nullptr;
+    w.fdev = rk ? (int*)(rk + 16) : nullptr;
+  }
   w.svd_bytes = svd_workspace_bytes(l, n);
   w.svdw = take(w.svd_bytes);
   w.sk_elems = max_of({splitk_workspace(l, m, n), splitk_workspace(l, n, m),
@@ -309,9 +313,13 @@ BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) 
   w.Rres = (double2*)take(Lg * Sh * 16);
   w.resid2 = (double2*)take(16);
   w.S = (double*)take(l * 8);
-  w.Ddev = (int64_t*)take(8);
-  w.edev = (double*)take(8);
-  w.fdev = (int*)take(8);
+  {
+    // D, discarded weight and fallback flag side by side: one read to the host
+    char* rk = (char*)take(32);
+    w.Ddev = (int64_t*)rk;
+    w.edev = rk ? (double*)(rk + 8) : nullptr;
+    w.fdev = rk ? (int*)(rk + 16) : nullptr;
+  }
   w.svd_bytes = svd_workspace_bytes(Lg, l, false);
   w.svdw = take(w.svd_bytes);
   // the factor products (D <= l columns) split when D = l fits
@@ -607,13 +615,16 @@ extern "C" int seemps_svd_split_sketch(int64_t m, int64_t n, const void* A_, int
                      w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
   if ((st = check_launch("truncate_sketch"))) return st;
   if ((st = svd_complete_enqueue(l, n, w.Ub, w.VHb, w.svdw, w.Ddev, st_))) return st;
+  char* hp = (char*)pinned_stage(PINNED_BYTES);
+  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
+  SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;  // synchronises
   int64_t hD = 0;
   double herr = 0.0;
   int hfb = 0;
-  SEEMPS_HIP(hipMemcpyAsync(&hD, w.Ddev, 8, hipMemcpyDeviceToHost, st_));
-  SEEMPS_HIP(hipMemcpyAsync(&herr, w.edev, 8, hipMemcpyDeviceToHost, st_));
-  SEEMPS_HIP(hipMemcpyAsync(&hfb, w.fdev, sizeof(int), hipMemcpyDeviceToHost, st_));
-  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;
+  memcpy(&hD, hp, 8);
+  memcpy(&herr, hp + 8, 8);
+  memcpy(&hfb, hp + 16, sizeof(int));
   if (hfb) return full_split();
   const int64_t D = hD;
   // U = Q U_B (m x l): (Q^T)^T U_B with Q^T stored l x m
@@ -761,13 +772,16 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
                      w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
   if ((st = check_launch("truncate_sketch"))) return st;
   if ((st = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return st;
+  char* hp = (char*)pinned_stage(PINNED_BYTES);
+  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
+  SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;  // synchronises
   int64_t hD = 0;
   double herr = 0.0;
   int hfb = 0;
-  SEEMPS_HIP(hipMemcpyAsync(&hD, w.Ddev, 8, hipMemcpyDeviceToHost, st_));
-  SEEMPS_HIP(hipMemcpyAsync(&herr, w.edev, 8, hipMemcpyDeviceToHost, st_));
-  SEEMPS_HIP(hipMemcpyAsync(&hfb, w.fdev, sizeof(int), hipMemcpyDeviceToHost, st_));
-  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;
+  memcpy(&hD, hp, 8);
+  memcpy(&herr, hp + 8, 8);
+  memcpy(&hfb, hp + 16, sizeof(int));
   if (hfb) return full_split();
   const int64_t D = hD;
   const double* sA = (mode == 1) ? w.S : nullptr;  // S into the left factor

@@ -1149,9 +1149,14 @@ int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t
 
 // Read the control block after the stream has been synchronised by the caller's copy.
 int svd_read_ctrl(void* work, int off, int32_t* info, hipStream_t stream) {
-  Ctrl h;
-  SEEMPS_HIP(hipMemcpyAsync(&h, (char*)work + off, sizeof(h), hipMemcpyDeviceToHost, stream));
+  static_assert(sizeof(Ctrl) <= PINNED_BYTES - PINNED_CTRL_OFF, "control block too large");
+  char* hp = (char*)pinned_stage(PINNED_BYTES);
+  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
+  hp += PINNED_CTRL_OFF;
+  SEEMPS_HIP(hipMemcpyAsync(hp, (char*)work + off, sizeof(Ctrl), hipMemcpyDeviceToHost, stream));
   SEEMPS_HIP(hipStreamSynchronize(stream));
+  Ctrl h;
+  memcpy(&h, hp, sizeof(h));
   if (info) {
     info[0] = h.info[0];
     info[1] = h.info[1];
@@ -1242,11 +1247,15 @@ int svd_split_impl(int64_t m, int64_t n, const double2* A, int mode, int method,
   if (st) return st;
   if ((st = truncate_enqueue(r, S, method, tolerance, max_bond, Ddev, edev, st_))) return st;
   if ((st = svd_complete_enqueue(m, n, U, VH, p, Ddev, st_))) return st;
+  char* hp = (char*)pinned_stage(PINNED_BYTES);
+  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
+  SEEMPS_HIP(hipMemcpyAsync(hp, Ddev, 8, hipMemcpyDeviceToHost, st_));
+  SEEMPS_HIP(hipMemcpyAsync(hp + 8, edev, 8, hipMemcpyDeviceToHost, st_));
+  if ((st = svd_read_ctrl(p, off, info, st_))) return st;  // synchronises
   int64_t hD = 0;
   double herr = 0.0;
-  SEEMPS_HIP(hipMemcpyAsync(&hD, Ddev, 8, hipMemcpyDeviceToHost, st_));
-  SEEMPS_HIP(hipMemcpyAsync(&herr, edev, 8, hipMemcpyDeviceToHost, st_));
-  if ((st = svd_read_ctrl(p, off, info, st_))) return st;
+  memcpy(&hD, hp, 8);
+  memcpy(&herr, hp + 8, 8);
   const int64_t D = hD;
   if (mode == 0) {
     if ((st = zscale2d_impl(m, D, nullptr, nullptr, U, r, outA, D, st_))) return st;
