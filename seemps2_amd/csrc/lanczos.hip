@@ -156,18 +156,28 @@ __device__ void lz_store_partials(const double2* acc, int k, double nrm, double2
   }
 }
 
-// h[j] = sum_b part[b * LZ_NV + j] (fixed order) into LDS, j < k.
+// h[j] = sum_b part[b * LZ_NV + j] into LDS, j < k (8 k <= blockDim.x): eight
+// lanes per j each add every eighth partial, then a three-step shuffle tree.
+// The order is fixed, so every block derives the same h.  Called by every
+// thread of the block.
 __device__ void lz_load_coeffs(int k, int nb, const double2* __restrict__ part, double2* hs) {
-  if ((int)threadIdx.x < k) {
-    double2 t = make_double2(0.0, 0.0);
-    for (int b = 0; b < nb; ++b) t = cadd(t, part[b * LZ_NV + threadIdx.x]);
-    hs[threadIdx.x] = t;
+  const int j = (int)threadIdx.x >> 3, sub = (int)threadIdx.x & 7;
+  double2 acc = make_double2(0.0, 0.0);
+  if (j < k)
+    for (int b = sub; b < nb; b += 8) acc = cadd(acc, part[b * LZ_NV + j]);
+  for (int off = 1; off < 8; off <<= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
   }
+  if (j < k && sub == 0) hs[j] = acc;
 }
 
-__device__ double lz_sum_norms(int nb, const double* __restrict__ pn) {
+// sum_b pn[b], b < nb, by one whole wavefront: lane l adds pn[l], pn[l + 64],
+// ..., then a butterfly, which leaves the same total (same order) in every lane.
+__device__ double lz_wave_sum(int nb, const double* __restrict__ pn) {
   double t = 0.0;
-  for (int b = 0; b < nb; ++b) t += pn[b];
+  for (int b = (int)threadIdx.x & 63; b < nb; b += 64) t += pn[b];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
   return t;
 }
 
@@ -249,8 +259,13 @@ __global__ __launch_bounds__(256) void lz_orth3(int64_t n, int k, const double2*
   __shared__ double s_nrm[2];
   __shared__ double2 red[4];
   const int nb = gridDim.x;
-  if (threadIdx.x == 0) s_nrm[0] = lz_sum_norms(nb, pn0);
-  if (threadIdx.x == 64) s_nrm[1] = lz_sum_norms(nb, pn1);
+  if (threadIdx.x < 64) {
+    const double t = lz_wave_sum(nb, pn0);
+    if (threadIdx.x == 0) s_nrm[0] = t;
+  } else if (threadIdx.x < 128) {
+    const double t = lz_wave_sum(nb, pn1);
+    if (threadIdx.x == 64) s_nrm[1] = t;
+  }
   lz_load_coeffs(k, nb, part_in, hs);
   __syncthreads();
   const bool second = !(s_nrm[1] >= 0.5 * s_nrm[0]);
@@ -288,9 +303,8 @@ __global__ __launch_bounds__(256) void lz_finish(int64_t n, int nparts, int j,
                                                  LzScalars* __restrict__ s, double2* __restrict__ w) {
   __shared__ double s_b;
   __shared__ int s_brk;
+  const double nrm2 = (threadIdx.x < 64) ? lz_wave_sum(nparts, pnorm) : 0.0;
   if (threadIdx.x == 0) {
-    double nrm2 = 0.0;
-    for (int b = 0; b < nparts; ++b) nrm2 += pnorm[b];
     const double a = s->h[j].x + s->h2[j].x;
     const double b = sqrt(fmax(nrm2, 0.0));
     const double prev = (j > 0) ? s->beta[j - 1] : 0.0;

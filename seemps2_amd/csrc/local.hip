@@ -123,13 +123,22 @@ __global__ __launch_bounds__(256) void zmdotc_partial(int64_t n, int nv,
 }
 
 // Pass 2: result[j] = sum_blk partial[blk * nv + j], fixed order.
-__global__ void zmdotc_final(int nblk, int nv, const double2* __restrict__ partial,
-                             double2* __restrict__ result) {
-  const int j = threadIdx.x;
-  if (j >= nv) return;
+// result[j] = sum_b partial[b * nv + j]: block j, thread t adds the partials
+// b = t, t + 256, ..., then a shuffle and LDS tree (a fixed order).  256 threads.
+__global__ __launch_bounds__(256) void zmdotc_final(int nblk, int nv,
+                                                    const double2* __restrict__ partial,
+                                                    double2* __restrict__ result) {
+  __shared__ double2 red[4];
+  const int j = blockIdx.x;
   double2 s = make_double2(0.0, 0.0);
-  for (int b = 0; b < nblk; ++b) s = cadd(s, partial[b * nv + j]);
-  result[j] = s;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s = cadd(s, partial[b * nv + j]);
+  for (int off = 32; off > 0; off >>= 1) {
+    s.x += __shfl_down(s.x, off, 64);
+    s.y += __shfl_down(s.y, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) result[j] = cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
 }
 
 }  // namespace
@@ -188,8 +197,8 @@ int zmdotc_impl(int64_t n, int64_t nv, const double2* V, int64_t ldv, const doub
                      ldv, w, work);
   int st = check_launch("zmdotc_partial");
   if (st) return st;
-  hipLaunchKernelGGL(zmdotc_final, dim3(1), dim3(64), 0, stream, (int)nblk, (int)nv, work,
-                     result);
+  hipLaunchKernelGGL(zmdotc_final, dim3((unsigned)nv), dim3(256), 0, stream, (int)nblk, (int)nv,
+                     work, result);
   return check_launch("zmdotc_final");
 }
 
