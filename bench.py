@@ -275,7 +275,10 @@ def timed_sweeps(QF, strategy, direction, steps, chi, precision="fp64"):
     return (time.perf_counter() - t0, t_chi, n_chi, n_all, D.STATS.matvecs, direction, E)
 
 
-TIMED_KERNELS = "svd_jacobi,zgemm_kernel,zgemm_kernel_fp32"
+# HIP-event timing inside the timed sweeps: the roofline kernel only, so the
+# events perturb the measured rate as little as possible (--kernel-timing
+# times more, e.g. "svd_jacobi,zgemm_kernel,zgemm_kernel_fp32" or "*")
+TIMED_KERNELS = "svd_jacobi"
 METRIC = "two-site bond updates/s at chi=256; speedup vs plain FP64 GPU at <=1e-9 rel err"
 
 

@@ -86,6 +86,18 @@ bool g_timing_all = false;
 std::vector<std::string> g_timing_filter;
 std::vector<TimingSlot> g_slots;
 std::vector<PendingTiming> g_pending;
+std::vector<hipEvent_t> g_free_events;  // drained events, reused by later launches
+
+// Caller holds g_timing_mu.  nullptr if no event can be created.
+hipEvent_t take_event() {
+  if (!g_free_events.empty()) {
+    hipEvent_t e = g_free_events.back();
+    g_free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
 
 // Caller holds g_timing_mu.
 int timing_slot(const char* name) {
@@ -116,8 +128,8 @@ int drain_pending() {
       set_error("kernel timing: event query failed");
       st = SEEMPS_ERR_RUNTIME;
     }
-    (void)hipEventDestroy(p.start);
-    (void)hipEventDestroy(p.stop);
+    g_free_events.push_back(p.start);
+    g_free_events.push_back(p.stop);
   }
   g_pending.clear();
   return st;
@@ -135,27 +147,29 @@ TimedLaunch::TimedLaunch(const char* name, hipStream_t stream, double flops)
   std::lock_guard<std::mutex> lock(g_timing_mu);
   const int slot = timing_slot(name);
   if (slot < 0) return;
-  if (hipEventCreate(&start_) != hipSuccess) return;
-  if (hipEventRecord(start_, stream_) != hipSuccess) {
-    (void)hipEventDestroy(start_);
+  hipEvent_t e = take_event();
+  if (!e) return;
+  if (hipEventRecord(e, stream_) != hipSuccess) {
+    g_free_events.push_back(e);
     return;
   }
+  start_ = e;
   slot_ = slot;
 }
 
 TimedLaunch::~TimedLaunch() {
   if (slot_ < 0) return;
-  hipEvent_t stop = nullptr;
-  if (hipEventCreate(&stop) != hipSuccess) {
-    (void)hipEventDestroy(start_);
+  std::lock_guard<std::mutex> lock(g_timing_mu);
+  hipEvent_t stop = take_event();
+  if (!stop) {
+    g_free_events.push_back(start_);
     return;
   }
   if (hipEventRecord(stop, stream_) != hipSuccess) {
-    (void)hipEventDestroy(start_);
-    (void)hipEventDestroy(stop);
+    g_free_events.push_back(start_);
+    g_free_events.push_back(stop);
     return;
   }
-  std::lock_guard<std::mutex> lock(g_timing_mu);
   g_pending.push_back(PendingTiming{slot_, start_, stop, flops_});
   // bound the number of live events on long runs
   if (g_pending.size() >= 8192) (void)drain_pending();
