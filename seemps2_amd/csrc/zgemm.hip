@@ -282,9 +282,16 @@ int dispatch(int prec, int opA, int opB, dim3 grid, hipStream_t stream, int64_t 
 
 // Number of K slices zgemm_ws uses for (M, N, K) (1 = no split).
 int64_t splitk_slices(int64_t M, int64_t N, int64_t K) {
+  // tuning knob: tile count from which no split is made (SEEMPS_SPLITK_TILES);
+  // the slices aim at twice that many workgroups
+  static const int64_t limit = [] {
+    const char* e = getenv("SEEMPS_SPLITK_TILES");
+    const long v = e ? atol(e) : 0;
+    return (int64_t)(v > 0 ? v : 192);
+  }();
   const int64_t tiles = ceil_div(M, BM) * ceil_div(N, BN);
-  if (tiles >= 192 || K < 256) return 1;
-  int64_t z = ceil_div(384, tiles);
+  if (tiles >= limit || K < 256) return 1;
+  int64_t z = ceil_div(2 * limit, tiles);
   if (z > 8) z = 8;
   while (z > 1 && K / z < 64) --z;
   return z;
