@@ -114,7 +114,17 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu, bool precondition = true) {
   s.nb = nb;
   s.ncolp = nb * BW;
   s.P = nb / 2;
+  // Row chunks per pair: one workgroup per CU (cooperative residency caps the
+  // grid there).  The QR-preconditioned matrix takes about half that while
+  // at least 8 chunks remain: fewer, longer chunks shorten each step's
+  // reduction and waits, and on 512 columns (16 pairs) R = 8 beat R = 16 over
+  // whole DMRG sweeps; on the basis-sketched matrices R = 16 stayed ahead.
   int64_t R = num_cu / s.P;
+  if (s.use_qr) {
+    int64_t half = num_cu / (2 * s.P);
+    if (half < 8) half = 8;
+    if (half < R) R = half;
+  }
   // tuning knob: cap on the row chunks per pair (SEEMPS_SVD_ROW_CHUNKS)
   static const int64_t r_cap = [] {
     const char* e = getenv("SEEMPS_SVD_ROW_CHUNKS");
