@@ -141,8 +141,10 @@ __device__ __forceinline__ void store_b(T* __restrict__ s_re, T* __restrict__ s_
 
 // Batch index z uses K (the last batch uses Klast: split-K slices share one
 // launch, the final slice being shorter).
+// At most 168 registers per lane (three waves per SIMD, no spills): the
+// default allocation (136 VGPRs + 64 AGPRs) left room for two.
 template <int OPA, int OPB, typename P>
-__global__ __launch_bounds__(256) void zgemm_kernel(
+__global__ __launch_bounds__(256, 3) void zgemm_kernel(
     int64_t M, int64_t N, int64_t Kfull, int64_t Klast, double2 alpha,
     const double2* __restrict__ A, int64_t lda, int64_t sA, const double2* __restrict__ B,
     int64_t ldb, int64_t sB, double2 beta, double2* __restrict__ C, int64_t ldc, int64_t sC) {
