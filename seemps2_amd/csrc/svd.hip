@@ -875,6 +875,17 @@ __global__ __launch_bounds__(1024) void svd_complete(SvdShape s, SvdWork w, doub
   const double tiny2 = w.ctrl->tiny2;
   int64_t kend = r;
   if (kmax && *kmax < kend) kend = *kmax;
+  // Usually no kept column is negligible: find out in parallel first, and
+  // leave without the sequential scan when none is.
+  __shared__ int s_any;
+  if (tid == 0) s_any = 0;
+  __syncthreads();
+  for (int64_t k = tid; k < kend; k += blockDim.x) {
+    const double sg = w.sigma[w.perm[k]];
+    if (!(sg > 0.0 && sg * sg > tiny2)) atomicOr(&s_any, 1);
+  }
+  __syncthreads();
+  if (!s_any) return;
   for (int64_t k = 0; k < kend; ++k) {
     const double sg = w.sigma[w.perm[k]];
     if (sg > 0.0 && sg * sg > tiny2) continue;
