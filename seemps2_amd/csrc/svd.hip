@@ -785,15 +785,26 @@ __global__ void svd_sigma(SvdShape s, SvdWork w) {
   }
 }
 
-// perm[rank] = c with descending sigma, ties by index.
-__global__ void svd_rank(SvdShape s, SvdWork w) {
+// perm[rank] = c with descending sigma, ties by index.  The keys are staged in
+// LDS (dynamic, ncolp doubles) when the launch provides it, else read from
+// memory.
+__global__ void svd_rank(SvdShape s, SvdWork w, int staged) {
+  extern __shared__ double keys[];
+  // NaN keys sort last, so the ranks always form a permutation
+  auto key = [&](int64_t d) -> double {
+    const double x = staged ? keys[d] : w.sigma[d];
+    return isnan(x) ? -1.0 : x;
+  };
+  if (staged) {
+    for (int64_t d = threadIdx.x; d < s.ncolp; d += blockDim.x) keys[d] = w.sigma[d];
+    __syncthreads();
+  }
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < s.ncolp;
        c += (int64_t)gridDim.x * blockDim.x) {
-    // NaN keys sort last, so the ranks always form a permutation
-    const double sc = isnan(w.sigma[c]) ? -1.0 : w.sigma[c];
+    const double sc = key(c);
     int rank = 0;
     for (int64_t d = 0; d < s.ncolp; ++d) {
-      const double sd = isnan(w.sigma[d]) ? -1.0 : w.sigma[d];
+      const double sd = key(d);
       rank += (sd > sc || (sd == sc && d < c)) ? 1 : 0;
     }
     w.perm[rank] = (int)c;
@@ -1116,7 +1127,12 @@ int svd_enqueue(int64_t m, int64_t n, const double2* A, int64_t lda, double2* U,
   hipLaunchKernelGGL(svd_sigma, dim3((unsigned)(s.ncolp < 1024 ? s.ncolp : 1024)), dim3(256), 0,
                      stream, s, w);
   if ((st = check_launch("svd_sigma"))) return st;
-  hipLaunchKernelGGL(svd_rank, dim3(blocks_for(s.ncolp)), dim3(256), 0, stream, s, w);
+  {
+    const int staged = s.ncolp <= 8192;  // up to 64 KB of keys per workgroup
+    const size_t shmem = staged ? (size_t)s.ncolp * sizeof(double) : 0;
+    hipLaunchKernelGGL(svd_rank, dim3(blocks_for(s.ncolp)), dim3(256), shmem, stream, s, w,
+                       staged);
+  }
   if ((st = check_launch("svd_rank"))) return st;
   if (s.use_qr) {
     hipLaunchKernelGGL(load_y, dim3(blocks_for(s.ncol * s.lenA)), dim3(256), 0, stream, s, w);
