@@ -143,12 +143,16 @@ SvdShape svd_shape(int64_t m, int64_t n, int num_cu, bool precondition = true) {
   // 31).  Measured over whole chi=256 DMRG sweeps this is faster in both the
   // QR-preconditioned and the basis-sketched splits (more sweeps at times,
   // each about half as long); the convergence test is the same either way.
-  // SEEMPS_SVD_INNER=full restores the full inner sweep at every step (tuning).
-  static const bool full_inner = [] {
+  // SEEMPS_SVD_INNER=full restores the full inner sweep at every step, and
+  // SEEMPS_SVD_INNER=hybrid keeps it for the first sweep only (tuning: hybrid
+  // converges the sketch matrices in fewer sweeps, but lost over DMRG sweeps).
+  static const int inner_mode = [] {
     const char* e = getenv("SEEMPS_SVD_INNER");
-    return e && strcmp(e, "full") == 0;
+    if (e && strcmp(e, "full") == 0) return 0;
+    if (e && strcmp(e, "hybrid") == 0) return 2;  // full inner sweeps in sweep 0
+    return 1;
   }();
-  s.cross_inner = (!full_inner && s.nb > 2) ? 1 : 0;
+  s.cross_inner = s.nb > 2 ? inner_mode : 0;
   return s;
 }
 
@@ -665,7 +669,7 @@ __global__ __launch_bounds__(256) void svd_jacobi(SvdShape s, SvdWork w, double 
     unsigned long long* off_slot = &w.ctrl->off[sweep % 3];
     for (int64_t k = 0; k < steps; ++k) {
       const int64_t g = (int64_t)sweep * steps + k;
-      const bool cross = s.cross_inner && k != 0;
+      const bool cross = s.cross_inner && k != 0 && (s.cross_inner == 1 || sweep > 0);
       double2* part = w.part + (g & 1) * part_stride;
       if (prof) t0 = now_ticks();
       if (LOCAL) {
