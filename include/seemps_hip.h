@@ -61,6 +61,12 @@ extern "C" {
 const char* seemps_last_error(void);
 /* ABI version of this header (incremented on incompatible changes). */
 int seemps_abi_version(void);
+/* Free the library's process-lifetime host resources: the pinned staging
+ * buffers of the device-to-host reads and the pooled timing events (pending
+ * timings are drained first).  Call it before the HIP runtime shuts down (the
+ * Python layer registers it with atexit).  The library stays usable: later
+ * calls allocate again.  No counterpart in the reference. */
+int seemps_release(void);
 
 /* ---------------------------------------------------------------------------
  * Contractions
@@ -138,7 +144,8 @@ int seemps_zgemm_grouped(int opA, int opB, int tile_cfg, int64_t nprob, const vo
 
 /* Local-operator contraction, complex128:
  *   out[p, j1, q, j2] = sum_i in[p, i, q] * Mop[i, j1, j2]   (+ out if accumulate)
- * with in[P, I, Q], Mop[I, J1, J2], out[P, J1, Q, J2] contiguous, 64*I + I*J1*J2 <= 3072.  Replaces the small-index steps of _mpo_multiply_tensor
+ * with in[P, I, Q], Mop[I, J1, J2], out[P, J1, Q, J2] contiguous (any sizes: the
+ * contracted index runs in LDS tiles, accumulated in order).  Replaces the small-index steps of _mpo_multiply_tensor
  * (src/seemps/operators/mpo.py:26-45, "cjd,aijb->caidb"), of the MPO environment
  * updates (state/environments.py:19-42), of the two-site matvec
  * (operators/quadratic.py:204-214) and of _contract_nrjl_ijk_klm

@@ -8,6 +8,7 @@ enqueued on torch's current stream.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
 from ctypes import c_double, c_int, c_int32, c_int64, c_void_p, POINTER
 from pathlib import Path
@@ -35,6 +36,7 @@ _D = c_double
 _SIGNATURES = {
     "seemps_last_error": (ctypes.c_char_p, []),
     "seemps_abi_version": (c_int, []),
+    "seemps_release": (c_int, []),
     "seemps_zgemm": (
         c_int,
         [c_int, c_int, _I, _I, _I, _D, _D, _P, _I, _I, _P, _I, _I, _D, _D, _P, _I, _I, _I, _P],
@@ -120,6 +122,11 @@ for _name, (_res, _args) in _SIGNATURES.items():
     _f.argtypes = _args
 
 ABI_VERSION = lib.seemps_abi_version()
+
+# Free the pinned staging buffers and pooled events while the HIP runtime is
+# still up (interpreter exit runs atexit handlers before the runtime's own
+# teardown).
+atexit.register(lib.seemps_release)
 
 
 def last_error() -> str:

@@ -10,12 +10,14 @@ passed in are uploaded first.  Also importable as ``seemps2_amd.cython``.
 
 from __future__ import annotations
 
+import ctypes
 import math
 from typing import Sequence
 
 import numpy as np
 import torch
 
+from . import _lib
 from . import kernels as K
 from ._lib import OP_C, OP_H, OP_N, OP_T
 
@@ -205,6 +207,7 @@ def _begin_environment(D: int = 1) -> torch.Tensor:
 
 def _update_left_environment(B, A, rho) -> torch.Tensor:
     """rho'[n,k] = sum B*[l,j,n] rho[l,i] A[i,j,k] (environments.pxi:10-32)."""
+    B, A, rho = K.as_dev(B), K.as_dev(A), K.as_dev(rho)
     if B.dim() != 3 or A.dim() != 3 or rho.dim() != 2:
         raise ValueError("Invalid arguments to _update_left_environment")
     l, j, n = B.shape
@@ -215,6 +218,7 @@ def _update_left_environment(B, A, rho) -> torch.Tensor:
 
 def _update_right_environment(B, A, rho) -> torch.Tensor:
     """rho'[i,l] = sum A[i,j,k] rho[k,n] B*[l,j,n] (environments.pxi:39-62)."""
+    B, A, rho = K.as_dev(B), K.as_dev(A), K.as_dev(rho)
     if B.dim() != 3 or A.dim() != 3 or rho.dim() != 2:
         raise ValueError("Invalid arguments to _update_right_environment")
     i, j, k = A.shape
@@ -225,11 +229,16 @@ def _update_right_environment(B, A, rho) -> torch.Tensor:
 
 def _end_environment(rho) -> complex:
     """Scalar product from the last environment (environments.pxi:64-66)."""
-    return complex(rho.reshape(-1)[0].item())
+    if isinstance(rho, torch.Tensor):
+        return complex(rho.reshape(-1)[0].item())
+    return complex(np.asarray(rho).reshape(-1)[0])
 
 
 def _join_environments(rhoL, rhoR) -> complex:
     """sum rhoL[a,b] rhoR[b,a] (environments.pxi:79-85)."""
+    rhoL, rhoR = K.as_dev(rhoL), K.as_dev(rhoR)
+    if rhoL.dim() != 2 or rhoR.dim() != 2 or rhoL.shape[::-1] != rhoR.shape:
+        raise ValueError("Invalid arguments to _join_environments")
     if rhoL.shape[0] == 1:
         return _end_environment(rhoL) * _end_environment(rhoR)
     return K.dotc(K.conj(rhoL), K.permute(rhoR, (1, 0)))
@@ -290,6 +299,27 @@ def _destructive_svd(A):
     return K.svd(A)
 
 
+def _shrink_ndarray(s: np.ndarray, D: int) -> None:
+    """Shorten the 1-D array ``s`` to its first ``D`` entries in place.
+
+    The reference sets the array's first dimension in place
+    (truncation.pxi:284-286), which also works for views that do not own their
+    data.  An owning array is resized by NumPy; for a view the dimension field
+    of the array object is rewritten the same way, after checking that the
+    object layout is the one expected (data pointer, ndim and length agree)."""
+    if s.flags.owndata and s.base is None:
+        s.resize(D, refcheck=False)
+        return
+    addr = id(s)
+    data = ctypes.c_void_p.from_address(addr + 16).value
+    nd = ctypes.c_int.from_address(addr + 24).value
+    dims = ctypes.c_void_p.from_address(addr + 32).value
+    if data != s.ctypes.data or nd != 1 or not dims or \
+            ctypes.c_ssize_t.from_address(dims).value != s.shape[0]:
+        raise TypeError("destructively_truncate_vector: cannot shrink this array in place")
+    ctypes.c_ssize_t.from_address(dims).value = D
+
+
 def destructively_truncate_vector(s, strategy: Strategy) -> float:
     """Truncate ``s`` in place, return the squared error (truncation.pxi:329-334)."""
     if isinstance(s, np.ndarray):
@@ -297,10 +327,10 @@ def destructively_truncate_vector(s, strategy: Strategy) -> float:
             raise TypeError("destructively_truncate_vector requires a 1D float64 ndarray")
         if s.size == 0:
             return 0.0
-        D, err = K.truncate(torch.as_tensor(s, dtype=torch.float64).to("cuda"),
+        D, err = K.truncate(torch.as_tensor(s, dtype=torch.float64).to(_lib.device()),
                             strategy.method, strategy.tolerance, strategy.max_bond_dimension)
         if D != s.shape[0]:
-            s.resize(D, refcheck=False)
+            _shrink_ndarray(s, D)
         return err
     if s.dim() != 1:
         raise TypeError("destructively_truncate_vector requires a 1D float64 tensor")

@@ -619,7 +619,6 @@ extern "C" int seemps_two_site_matvec(int64_t a, int64_t c, int64_t b, int64_t d
                                       void* work, int64_t work_bytes, void* stream) {
   SEEMPS_CHECK_ARG(work_bytes >= seemps_two_site_matvec_workspace(a, c, b, d1, d2, w2, f),
                    "two_site_matvec: workspace too small");
-  SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "two_site_matvec: w*d1*d2 > 64 not supported");
   double2* x1 = (double2*)work;
   double2* x2 = (double2*)((char*)work + al256(a * c * d1 * d2 * f * 16));
   double2* ks = (double2*)((char*)x2 + al256(a * d1 * d2 * w2 * f * 16));
@@ -650,7 +649,6 @@ int eigensolve_impl(int prec, int64_t chiL, int64_t c, int64_t d1, int64_t d2, i
                    "eigensolve: bad dimensions");
   SEEMPS_CHECK_ARG(ncv >= 2 && ncv < LZ_KREG, "eigensolve: ncv must be in [2, %d]",
                    LZ_KREG - 1);
-  SEEMPS_CHECK_ARG(c * d1 * d2 <= 64, "eigensolve: w*d1*d2 > 64 not supported");
   SEEMPS_CHECK_ARG(maxiter >= 1, "eigensolve: maxiter must be positive");
   const int64_t n = chiL * d1 * d2 * chiR;
   const int64_t m = ncv < n ? ncv : n;

@@ -8,34 +8,43 @@ namespace {
 
 constexpr int LQ = 64;  // q-values per workgroup of zlocal
 
-// out[p, j1, q, j2] (+)= sum_i in[p, i, q] Mop[i, j1, j2].
-// One workgroup handles (p, 64 consecutive q); the I x 64 input slab and the
-// whole I x J operator are staged in LDS, 4 threads per q split the J outputs.
+// out[p, j1, q, j2] (+)= sum_{i0 <= i < i0 + IC} in[p, i, q] Mop[i, j1, j2] for the
+// output columns j = j1 * J2 + j2 in [j0, j0 + JC), j0 = blockIdx.y * JC.
+// One workgroup handles (p, 64 consecutive q, one j tile); the IC x 64 input
+// slab and the IC x JC operator tile are staged in LDS, 4 threads per q split
+// the tile's outputs.  The host runs the i tiles one after another, the later
+// ones accumulating, so a contracted index of any size fits in LDS.
 __global__ __launch_bounds__(256) void zlocal_kernel(int64_t P, int I, int64_t Q, int J1, int J2,
-                                                     int64_t nqb,
+                                                     int64_t nqb, int i0, int IC, int JC,
                                                      const double2* __restrict__ in,
                                                      const double2* __restrict__ Mop,
                                                      double2* __restrict__ out, int accumulate) {
   extern __shared__ double2 smem[];
   const int J = J1 * J2;
-  double2* xs = smem;           // [I][LQ]
-  double2* ms = smem + I * LQ;  // [I][J]
+  const int j0 = blockIdx.y * JC;
+  const int jn = min(JC, J - j0);
+  double2* xs = smem;            // [IC][LQ]
+  double2* ms = smem + IC * LQ;  // [IC][JC]
   const int64_t p = blockIdx.x / nqb;
   const int64_t q0 = (blockIdx.x % nqb) * LQ;
   const int tid = threadIdx.x;
-  for (int e = tid; e < I * LQ; e += blockDim.x) {
+  for (int e = tid; e < IC * LQ; e += blockDim.x) {
     const int i = e / LQ, ql = e % LQ;
     const int64_t q = q0 + ql;
-    xs[e] = (q < Q) ? in[(p * I + i) * Q + q] : make_double2(0.0, 0.0);
+    xs[e] = (q < Q) ? in[(p * I + i0 + i) * Q + q] : make_double2(0.0, 0.0);
   }
-  for (int e = tid; e < I * J; e += blockDim.x) ms[e] = Mop[e];
+  for (int e = tid; e < IC * jn; e += blockDim.x) {
+    const int i = e / jn, jl = e % jn;
+    ms[i * JC + jl] = Mop[(int64_t)(i0 + i) * J + j0 + jl];
+  }
   __syncthreads();
   const int ql = tid % LQ;
   const int64_t q = q0 + ql;
   if (q >= Q) return;
-  for (int j = tid / LQ; j < J; j += blockDim.x / LQ) {
+  for (int jl = tid / LQ; jl < jn; jl += blockDim.x / LQ) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int i = 0; i < I; ++i) acc = cfma(xs[i * LQ + ql], ms[i * J + j], acc);
+    for (int i = 0; i < IC; ++i) acc = cfma(xs[i * LQ + ql], ms[i * JC + jl], acc);
+    const int j = j0 + jl;
     const int j1 = j / J2, j2 = j % J2;
     const int64_t o = ((p * J1 + j1) * Q + q) * J2 + j2;
     if (accumulate) acc = cadd(acc, out[o]);
@@ -153,18 +162,34 @@ int64_t reduce_blocks(int64_t n) {
 int zlocal_impl(int64_t P, int64_t I, int64_t Q, int64_t J1, int64_t J2, const double2* in,
                 const double2* Mop, double2* out, int accumulate, hipStream_t stream) {
   SEEMPS_CHECK_ARG(P >= 0 && I >= 1 && Q >= 0 && J1 >= 1 && J2 >= 1, "zlocal: bad dimensions");
-  SEEMPS_CHECK_ARG(I <= 64, "zlocal: contracted dimension %lld > 64", (long long)I);
-  SEEMPS_CHECK_ARG(I * LQ + I * J1 * J2 <= 3072, "zlocal: operator too large (I=%lld, J=%lld)",
-                   (long long)I, (long long)(J1 * J2));
+  SEEMPS_CHECK_ARG(I <= 0x7fffffff && J1 * J2 <= 0x7fffffff, "zlocal: dimensions too large");
   if (P == 0 || Q == 0) return SEEMPS_OK;
+  const int64_t J = J1 * J2;
+  // LDS tiles: IC x 64 inputs and IC x JC operator entries, at most 3072
+  // complex values (48 KiB).  When the whole operator fits (the MPO and
+  // physical indices of d = 2 chains) one launch does everything; otherwise
+  // the contracted index runs in tiles of 32 and the outputs in tiles of 32.
+  int64_t IC = I, JC = J;
+  if (I * LQ + I * J > 3072) {
+    IC = I < 32 ? I : 32;
+    JC = (3072 - IC * LQ) / IC;
+    if (JC > J) JC = J;
+  }
   const int64_t nqb = ceil_div(Q, LQ);
   const int64_t nblocks = P * nqb;
-  SEEMPS_CHECK_ARG(nblocks <= 0x7fffffff, "zlocal: grid too large");
-  const size_t shmem = (size_t)(I * LQ + I * J1 * J2) * sizeof(double2);
-  TimedLaunch timed("zlocal_kernel", stream, 8.0 * P * I * Q * J1 * J2);
-  hipLaunchKernelGGL(zlocal_kernel, dim3((unsigned)nblocks), dim3(256), shmem, stream, P, (int)I,
-                     Q, (int)J1, (int)J2, nqb, in, Mop, out, accumulate);
-  return check_launch("zlocal");
+  const int64_t njb = ceil_div(J, JC);
+  SEEMPS_CHECK_ARG(nblocks <= 0x7fffffff && njb <= 65535, "zlocal: grid too large");
+  const size_t shmem = (size_t)(IC * LQ + IC * JC) * sizeof(double2);
+  TimedLaunch timed("zlocal_kernel", stream, 8.0 * P * I * Q * J);
+  for (int64_t i0 = 0; i0 < I; i0 += IC) {
+    const int64_t ic = (I - i0) < IC ? (I - i0) : IC;
+    hipLaunchKernelGGL(zlocal_kernel, dim3((unsigned)nblocks, (unsigned)njb), dim3(256), shmem,
+                       stream, P, (int)I, Q, (int)J1, (int)J2, nqb, (int)i0, (int)ic, (int)JC,
+                       in, Mop, out, (i0 > 0) ? 1 : accumulate);
+    const int st = check_launch("zlocal");
+    if (st) return st;
+  }
+  return SEEMPS_OK;
 }
 
 int zaxpby_impl(int64_t n, double2 a, const double2* x, int conj_x, double2 b, double2* y,

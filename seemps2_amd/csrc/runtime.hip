@@ -31,16 +31,37 @@ int check_launch(const char* what) {
 // Pinned host staging for the small device-to-host reads of the hot paths
 // (eigensolver exits, split ranks, SVD control blocks): copies into pinned
 // memory are plain DMA, where copies into pageable memory go through a
-// runtime bounce buffer.  One buffer per host thread, allocated on first use
-// and kept for the process; a request larger than the buffer replaces it.
+// runtime bounce buffer.  One buffer per host thread, allocated on first use;
+// a request larger than the buffer replaces it.  Every buffer is listed in a
+// registry so that seemps_release() can free them all; it bumps a generation
+// number, and a thread whose buffer belongs to an older generation allocates
+// a new one on its next use.
+namespace {
+std::mutex g_pin_mu;
+std::vector<void*> g_pin_all;
+std::atomic<int> g_pin_gen{0};
+}  // namespace
+
 void* pinned_stage(size_t bytes) {
   static thread_local void* buf = nullptr;
   static thread_local size_t cap = 0;
+  static thread_local int gen = -1;
+  const int now = g_pin_gen.load(std::memory_order_acquire);
+  if (gen != now) {  // released (or never allocated) since this thread's last use
+    buf = nullptr;
+    cap = 0;
+    gen = now;
+  }
   if (bytes > cap) {
     const size_t want = std::max<size_t>(bytes, 4096);
     void* p = nullptr;
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
-    if (buf) (void)hipHostFree(buf);
+    std::lock_guard<std::mutex> lock(g_pin_mu);
+    if (buf) {
+      (void)hipHostFree(buf);
+      g_pin_all.erase(std::remove(g_pin_all.begin(), g_pin_all.end(), buf), g_pin_all.end());
+    }
+    g_pin_all.push_back(p);
     buf = p;
     cap = want;
   }
@@ -178,6 +199,26 @@ TimedLaunch::~TimedLaunch() {
 }  // namespace seemps
 
 extern "C" const char* seemps_last_error(void) { return seemps::g_last_error; }
+
+extern "C" int seemps_release(void) {
+  using namespace seemps;
+  int st = SEEMPS_OK;
+  {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    st = drain_pending();
+    for (hipEvent_t e : g_free_events) (void)hipEventDestroy(e);
+    g_free_events.clear();
+    g_slots.clear();
+    g_timing_on.store(false);
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_pin_mu);
+    for (void* p : g_pin_all) (void)hipHostFree(p);
+    g_pin_all.clear();
+    g_pin_gen.fetch_add(1, std::memory_order_acq_rel);
+  }
+  return st;
+}
 
 extern "C" int seemps_abi_version(void) { return 1; }
 

@@ -13,6 +13,9 @@
 #include <cmath>
 #include <initializer_list>
 
+#include <cstdlib>
+#include <cstring>
+
 #include "common.h"
 #include "jacobi_inner.h"
 
@@ -321,6 +324,7 @@ BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) 
     w.fdev = rk ? (int*)(rk + 16) : nullptr;
   }
   w.svd_bytes = svd_workspace_bytes(Lg, l, false);
+  if (polar_svd_bytes(Lg, l) > w.svd_bytes) w.svd_bytes = polar_svd_bytes(Lg, l);
   w.svdw = take(w.svd_bytes);
   // the factor products (D <= l columns) split when D = l fits
   w.sk_elems = max_of({splitk_workspace(pp, Lg, Sh), splitk_workspace(pp, Sh, Lg),
@@ -749,11 +753,25 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
     }
   }
   int off = 0;
+  // W has nearly orthogonal columns: by default its SVD is the simultaneous
+  // rotation iteration of polar_svd.hip (a few GEMM iterations); the block
+  // Jacobi (SEEMPS_BASIS_SVD=jacobi) is the alternative.  A polar iteration
+  // that does not converge hands the split to the full SVD.
+  static const bool use_polar = [] {
+    const char* e = getenv("SEEMPS_BASIS_SVD");
+    return !(e && strcmp(e, "jacobi") == 0);
+  }();
+  const int64_t* tiny_dev = nullptr;
   {
     TimedLaunch phase("phase_basis_svd", st_, 0.0);
-    if ((st = svd_enqueue(Lg, l, w.W, l, w.Uw, w.S, w.VHw, w.svdw, w.svd_bytes, st_, &off, false,
-                          false)))
+    if (use_polar) {
+      st = polar_svd(Lg, l, w.W, l, w.Uw, w.S, w.VHw, &tiny_dev, w.svdw, w.svd_bytes, info, st_);
+      if (st == SEEMPS_ERR_NOCONVERGE) return full_split();
+      if (st) return st;
+    } else if ((st = svd_enqueue(Lg, l, w.W, l, w.Uw, w.S, w.VHw, w.svdw, w.svd_bytes, st_, &off,
+                                 false, false))) {
       return st;
+    }
   }
   if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
   // R = A - (projection of A on Omega): side 0: A - W Oh; side 1: A - Oh^H W^H
@@ -771,18 +789,28 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
   hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
                      w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
   if ((st = check_launch("truncate_sketch"))) return st;
-  if ((st = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return st;
   char* hp = (char*)pinned_stage(PINNED_BYTES);
   SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
-  SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
-  if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;  // synchronises
+  int64_t htiny = l;
+  if (use_polar) {
+    SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+    SEEMPS_HIP(hipMemcpyAsync(hp + 24, tiny_dev, 8, hipMemcpyDeviceToHost, st_));
+    SEEMPS_HIP(hipStreamSynchronize(st_));
+    memcpy(&htiny, hp + 24, 8);
+  } else {
+    if ((st = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return st;
+    SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+    if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;  // synchronises
+  }
   int64_t hD = 0;
   double herr = 0.0;
   int hfb = 0;
   memcpy(&hD, hp, 8);
   memcpy(&herr, hp + 8, 8);
   memcpy(&hfb, hp + 16, sizeof(int));
-  if (hfb) return full_split();
+  // a kept slot with a negligible singular value has no vector from the
+  // rotation iteration: the full SVD (with its orthonormal completion) decides
+  if (hfb || htiny < hD) return full_split();
   const int64_t D = hD;
   const double* sA = (mode == 1) ? w.S : nullptr;  // S into the left factor
   const double* sB = (mode == 0) ? w.S : nullptr;  // S into the right factor

@@ -984,15 +984,21 @@ __global__ __launch_bounds__(1024) void svd_complete(SvdShape s, SvdWork w, doub
   }
 }
 
-// Truncation rule: one lane runs the reference's sequential loops over an LDS
-// copy of s (summation order identical to truncation.pxi:251-327, products and
-// sums rounded separately as in the reference's compiled loops).
+// Truncation rule: one lane runs the reference's sequential loops (summation
+// order identical to truncation.pxi:251-327, products and sums rounded
+// separately as in the reference's compiled loops) over an LDS copy of s
+// (STAGED, spectra of up to TRUNC_STAGE values) or over s itself.
+constexpr int64_t TRUNC_STAGE = 4096;
+template <bool STAGED>
 __global__ void truncate_kernel(int64_t N, const double* __restrict__ s, int method, double tol,
                                 int64_t max_bond, int64_t* D_out, double* err_out) {
 #pragma clang fp contract(off)
-  extern __shared__ double ss[];
-  for (int64_t i = threadIdx.x; i < N; i += blockDim.x) ss[i] = s[i];
-  __syncthreads();
+  extern __shared__ double stage[];
+  if (STAGED) {
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) stage[i] = s[i];
+    __syncthreads();
+  }
+  const double* ss = STAGED ? stage : s;
   if (threadIdx.x != 0) return;
   int64_t D = N;
   double err = 0.0;
@@ -1180,11 +1186,14 @@ int64_t svd_workspace_bytes(int64_t m, int64_t n, bool precondition) {
 int truncate_enqueue(int64_t N, const double* s, int method, double tol, int64_t max_bond,
                      int64_t* D_out, double* err_out, hipStream_t stream) {
   SEEMPS_CHECK_ARG(N >= 1, "truncate: empty spectrum");
-  SEEMPS_CHECK_ARG(N <= 4096, "truncate: spectrum longer than 4096");
   SEEMPS_CHECK_ARG(method >= 0 && method <= 3, "truncate: invalid method %d", method);
   SEEMPS_CHECK_ARG(max_bond >= 1, "truncate: invalid max bond dimension");
-  hipLaunchKernelGGL(truncate_kernel, dim3(1), dim3(256), (size_t)N * sizeof(double), stream, N,
-                     s, method, tol, max_bond, D_out, err_out);
+  if (N <= TRUNC_STAGE)
+    hipLaunchKernelGGL(truncate_kernel<true>, dim3(1), dim3(256), (size_t)N * sizeof(double),
+                       stream, N, s, method, tol, max_bond, D_out, err_out);
+  else
+    hipLaunchKernelGGL(truncate_kernel<false>, dim3(1), dim3(64), 0, stream, N, s, method, tol,
+                       max_bond, D_out, err_out);
   return check_launch("truncate");
 }
 

@@ -1,9 +1,24 @@
-"""Nearest-neighbour Hamiltonians (reference: src/seemps/hamiltonians/nearest_neighbor.py).
+"""Nearest-neighbour Hamiltonians H = sum_i h_{i,i+1} and their MPOs.
 
-``to_mpo`` follows nearest_neighbor.py:105-165: each two-site term is split by
-an SVD (on the device, truncated with the Strategy rule) into the Schmidt
-channels of an MPO with bond dimension 2 + rank(h).  The small MPO tensors
-are assembled on the host from the device SVD factors and uploaded once.
+Drop-in for ``seemps.hamiltonians`` (reference: src/seemps/hamiltonians/
+nearest_neighbor.py:16-334): same classes, constructor arguments and methods.
+
+MPO channel layout (the operator a user of the reference gets, kept as an
+interface).  The bond between sites i and i+1 carries 2 + r_i channels, r_i
+being the rank of h_{i,i+1} seen as a matrix between the operator spaces of
+the two sites, h = sum_k A_k (x) B_k:
+
+* channel 0: no term has been placed to the left of the bond yet;
+* channel 1: one term has been placed (and completed) to the left;
+* channel 2 + k: term k of the bond has been opened on site i by A_k and is
+  closed on site i+1 by B_k.
+
+A site tensor W[a, out, in, b] therefore holds the identity on (0 -> 0) and
+(1 -> 1), A_k on (0 -> 2 + k) towards its right bond and B_k on (2 + k -> 1)
+from its left bond.  The first site keeps row channel 0 only, the last site
+column channel 1 only.  The split h = sum_k A_k (x) B_k is the Schmidt
+decomposition of h (device SVD, truncated with the Strategy rule), with
+sqrt(s_k) on each side.
 """
 
 from __future__ import annotations
@@ -21,11 +36,52 @@ from ..operators.mpo import MPO
 
 SX = np.array([[0.0, 1.0], [1.0, 0.0]])
 SZ = np.array([[1.0, 0.0], [0.0, -1.0]])
-SY = -1j * SZ @ SX
+SY = np.array([[0.0, -1j], [1j, 0.0]])
+
+
+def _schmidt_terms(h, d1: int, d2: int, strategy: Strategy):
+    """Operator Schmidt decomposition of the two-site term h (d1 d2 x d1 d2).
+
+    Returns (A, B) with A[k] (d1 x d1) and B[k] (d2 x d2) such that
+    h = sum_k kron(A[k], B[k]), the weights split as sqrt(s_k) on each side."""
+    dense = h.toarray() if sp.issparse(h) else np.asarray(h)
+    # h[(o1, o2), (i1, i2)] -> M[(o1, i1), (o2, i2)]
+    M = dense.reshape(d1, d2, d1, d2).transpose(0, 2, 1, 3).reshape(d1 * d1, d2 * d2)
+    U, s, VH = K.svd(K.as_dev(np.ascontiguousarray(M)))
+    rank, _ = K.truncate(s, strategy.method, strategy.tolerance, strategy.max_bond_dimension)
+    w = np.sqrt(s.cpu().numpy()[:rank])
+    U = K.to_numpy(U)[:, :rank] * w
+    VH = w[:, None] * K.to_numpy(VH)[:rank, :]
+    return U.T.reshape(rank, d1, d1), VH.reshape(rank, d2, d2)
+
+
+def _site_tensor(d: int, closing, opening, first: bool, last: bool) -> np.ndarray:
+    """W[a, out, in, b] of one site in the channel layout of the module.
+
+    ``closing`` (r_left x d x d or None) are the B_k of the bond on the left,
+    ``opening`` (r_right x d x d or None) the A_k of the bond on the right."""
+    r_left = 0 if closing is None else closing.shape[0]
+    r_right = 0 if opening is None else opening.shape[0]
+    dtype = np.result_type(np.float64, *(x.dtype for x in (closing, opening) if x is not None))
+    W = np.zeros((2 + r_left, d, d, 2 + r_right), dtype=dtype)
+    identity = np.eye(d)
+    W[0, :, :, 0] = identity
+    W[1, :, :, 1] = identity
+    for k in range(r_right):
+        W[0, :, :, 2 + k] = opening[k]
+    for k in range(r_left):
+        W[2 + k, :, :, 1] = closing[k]
+    if first:
+        W = W[:1]
+    if last:
+        W = W[:, :, :, 1:2]
+    return np.ascontiguousarray(W)
 
 
 class NNHamiltonian(ABC):
-    """H = sum_i h_{i,i+1} on a chain (nearest_neighbor.py:16-66)."""
+    """H = sum_i h_{i,i+1} on a chain of ``size`` subsystems
+    (nearest_neighbor.py:16-66).  Subclasses give :meth:`dimension` and
+    :meth:`interaction_term`."""
 
     def __init__(self, size: int, constant: bool = False):
         self.size = size
@@ -33,12 +89,12 @@ class NNHamiltonian(ABC):
 
     @abstractmethod
     def dimension(self, i: int) -> int:
-        pass
+        """Physical dimension of subsystem ``i``."""
 
     def interaction_term(self, i: int, t: float = 0.0):
-        d1 = self.dimension(i)
-        d2 = self.dimension(i + 1)
-        return sp.csr_matrix((d1 * d2, d1 * d2))
+        """h_{i,i+1} at time ``t``; zero unless a subclass says otherwise."""
+        n = self.dimension(i) * self.dimension(i + 1)
+        return sp.csr_matrix((n, n))
 
     def tomatrix(self, t: float = 0.0):
         warnings.warn("Method Hamiltonian.tomatrix() has been renamed to_matrix()",
@@ -46,83 +102,71 @@ class NNHamiltonian(ABC):
         return self.to_matrix(t)
 
     def to_matrix(self, t: float = 0.0):
-        """Sparse matrix (nearest_neighbor.py:77-103)."""
-        dleft = 1
-        H = sp.bsr_matrix((self.dimension(0), self.dimension(0)))
+        """Sparse matrix sum_i 1_{<i} (x) h_{i,i+1} (x) 1_{>i+1}."""
+        dims = [self.dimension(i) for i in range(self.size)]
+        total = int(np.prod(dims))
+        H = sp.csr_matrix((total, total))
         for i in range(self.size - 1):
-            H = sp.kron(H, sp.eye(self.dimension(i + 1)))
-            H = H + sp.kron(sp.eye(dleft if dleft else 1), self.interaction_term(i, t))
-            dleft *= self.dimension(i)
-        return H.tocsr()
+            left = int(np.prod(dims[:i]))
+            right = int(np.prod(dims[i + 2:]))
+            term = sp.csr_matrix(self.interaction_term(i, t))
+            H = H + sp.kron(sp.kron(sp.identity(left), term), sp.identity(right))
+        return sp.csr_matrix(H)
 
     def to_mpo(self, t: float = 0.0, strategy: Strategy = DEFAULT_STRATEGY) -> MPO:
-        """MPO with bond 2 + rank(h_{i,i+1}) (nearest_neighbor.py:105-165)."""
-        tensors = [np.zeros((2, di, di, 2)) for i in range(self.size)
-                   for di in [self.dimension(i)]]
-        for i in range(self.size - 1):
-            Hij = np.asarray(sp.csr_matrix(self.interaction_term(i, t)).toarray())
-            di = self.dimension(i)
-            dj = self.dimension(i + 1)
-            Hij = Hij.reshape(di, dj, di, dj).transpose(0, 2, 1, 3).reshape(di * di, dj * dj)
-            U, s, V = K.svd(K.as_dev(Hij))
-            ds, _ = K.truncate(s, strategy.method, strategy.tolerance,
-                               strategy.max_bond_dimension)
-            U, s, V = K.to_numpy(U)[:, :ds], s.cpu().numpy()[:ds], K.to_numpy(V)[:ds, :]
-            s = np.sqrt(s)
-            A = tensors[i]
-            a, j, k, b = A.shape
-            B = np.zeros((a, j, k, b + ds), dtype=np.result_type(U, A))
-            B[:, :, :, :b] = A
-            B[0, :, :, 2:] = (U * s).reshape(di, di, ds)
-            B[1, :, :, 1] = np.eye(j)
-            B[0, :, :, 0] = np.eye(j)
-            tensors[i] = B
-            A = tensors[i + 1]
-            a, j, k, b = A.shape
-            B = np.zeros((a + ds, j, k, b), dtype=np.result_type(V, A))
-            B[:a, :, :, :] = A
-            B[2:, :, :, 1] = (s.reshape(ds, 1) * V).reshape(ds, dj, dj)
-            B[1, :, :, 1] = np.eye(j)
-            B[0, :, :, 0] = np.eye(j)
-            tensors[i + 1] = B
-        tensors[0] = tensors[0][[0], :, :, :]
-        tensors[-1] = tensors[-1][:, :, :, [1]]
+        """MPO in the channel layout of the module, bond 2 + rank(h_{i,i+1})
+        (nearest_neighbor.py:105-165)."""
+        N = self.size
+        terms = [_schmidt_terms(self.interaction_term(i, t), self.dimension(i),
+                                self.dimension(i + 1), strategy) for i in range(N - 1)]
+        tensors = []
+        for i in range(N):
+            closing = terms[i - 1][1] if i > 0 else None
+            opening = terms[i][0] if i < N - 1 else None
+            tensors.append(_site_tensor(self.dimension(i), closing, opening, i == 0,
+                                        i == N - 1))
         return MPO(tensors)
 
 
 class ConstantNNHamiltonian(NNHamiltonian):
-    """Nearest-neighbour Hamiltonian with constant terms (nearest_neighbor.py:168-280)."""
+    """Nearest-neighbour Hamiltonian with time-independent terms
+    (nearest_neighbor.py:168-280).  ``dimension`` is one int for every site or
+    a list of ``size`` ints."""
 
     def __init__(self, size: int, dimension):
         super().__init__(size, True)
         if isinstance(dimension, list):
             if len(dimension) != size:
                 raise ValueError("len(dimension) must equal the Hamiltonian size")
-            self.dimensions = dimension
+            self.dimensions = list(dimension)
         else:
             self.dimensions = [dimension] * size
-        self.interactions = [np.zeros((si * sj, si * sj))
-                             for si, sj in zip(self.dimensions[:-1], self.dimensions[1:])]
+        self.interactions = [np.zeros((self.dimensions[i] * self.dimensions[i + 1],) * 2)
+                             for i in range(size - 1)]
 
     def add_local_term(self, site: int, operator) -> "ConstantNNHamiltonian":
-        if site < 0 or site >= self.size:
-            raise IndexError("Site {site} out of bounds in add_local_term()")
-        if site == 0:
-            self.add_interaction_term(site, operator, np.eye(self.dimensions[1]))
-        elif site == self.size - 1:
-            self.add_interaction_term(site - 1, np.eye(self.dimensions[site - 1]), operator)
-        else:
-            half = 0.5 * sp.csr_matrix(operator)
-            self.add_interaction_term(site - 1, np.eye(self.dimensions[site - 1]), half)
-            self.add_interaction_term(site, half, np.eye(self.dimensions[site + 1]))
+        """Add ``operator`` acting on ``site``.  It is shared evenly between the
+        bonds that touch the site (one bond at either end of the chain, two in
+        the bulk)."""
+        bonds = [b for b in (site - 1, site) if 0 <= b < self.size - 1]
+        if not (0 <= site < self.size) or not bonds:
+            raise IndexError(f"Site {site} out of bounds in add_local_term()")
+        weight = 1.0 / len(bonds)
+        part = weight * sp.csr_matrix(operator) if weight != 1.0 else operator
+        for b in bonds:
+            if b == site:   # the site is the left member of the bond
+                self.add_interaction_term(b, part, np.eye(self.dimensions[b + 1]))
+            else:           # the right member
+                self.add_interaction_term(b, np.eye(self.dimensions[b]), part)
         return self
 
     def add_interaction_term(self, i: int, op1, op2=None) -> "ConstantNNHamiltonian":
-        if i < 0 or i >= self.size - 1:
-            raise IndexError("Site {site} out of bounds in add_interaction_term()")
+        """Add ``op1`` (a two-site operator) or ``op1 (x) op2`` to h_{i,i+1}."""
+        if not (0 <= i < self.size - 1):
+            raise IndexError(f"Site {i} out of bounds in add_interaction_term()")
         H12 = op1 if op2 is None else sp.kron(op1, op2)
-        if (H12.ndim != 2 or H12.shape[0] != H12.shape[1]
-                or H12.shape[1] != self.dimension(i) * self.dimension(i + 1)):
+        n = self.dimension(i) * self.dimension(i + 1)
+        if H12.ndim != 2 or H12.shape != (n, n):
             raise ValueError("Invalid operators supplied to add_interaction_term()")
         self.interactions[i] = self.interactions[i] + H12
         return self
@@ -135,7 +179,9 @@ class ConstantNNHamiltonian(NNHamiltonian):
 
 
 class ConstantTIHamiltonian(ConstantNNHamiltonian):
-    """Translationally invariant chain (nearest_neighbor.py:283-315)."""
+    """Translationally invariant chain (nearest_neighbor.py:283-315): the same
+    ``interaction`` on every bond and ``local_term`` on sites 0 .. size-2 (the
+    reference's loop over bonds leaves the last site without it)."""
 
     def __init__(self, size: int, interaction=None, local_term=None):
         if local_term is not None:
@@ -145,26 +191,27 @@ class ConstantTIHamiltonian(ConstantNNHamiltonian):
         else:
             raise ValueError("Either interactions or local term must be supplied")
         super().__init__(size, dimension)
-        for site in range(size - 1):
+        for bond in range(size - 1):
             if interaction is not None:
-                self.add_interaction_term(site, interaction)
+                self.add_interaction_term(bond, interaction)
             if local_term is not None:
-                self.add_local_term(site, local_term)
+                self.add_local_term(bond, local_term)
 
 
 class HeisenbergHamiltonian(ConstantTIHamiltonian):
-    """0.25 sigma.sigma between neighbouring spins 1/2 (nearest_neighbor.py:318-334)."""
+    """(1/4) sigma . sigma between neighbouring spins 1/2, plus an optional
+    field (hx, hy, hz) . sigma (nearest_neighbor.py:318-334)."""
 
     def __init__(self, size: int, field=None):
-        Hint = 0.25 * (sp.kron(SX, SX) + sp.kron(SY, SY) + sp.kron(SZ, SZ)).real
-        Hlocal = None if field is None else field[0] * SX + field[1] * SY + field[2] * SZ
-        super().__init__(size, interaction=Hint, local_term=Hlocal)
+        exchange = 0.25 * sum(sp.kron(P, P) for P in (SX, SY, SZ)).real
+        local = None if field is None else sum(f * P for f, P in zip(field, (SX, SY, SZ)))
+        super().__init__(size, interaction=exchange, local_term=local)
 
 
 class XXZHamiltonian(ConstantTIHamiltonian):
     """Jxy (Sx Sx + Sy Sy) + Jz Sz Sz with S = sigma / 2 (open XXZ chain)."""
 
     def __init__(self, size: int, Jxy: float = 1.0, Jz: float = 0.5):
-        Hint = (Jxy * 0.25 * (np.kron(SX, SX) + np.kron(SY, SY))
-                + Jz * 0.25 * np.kron(SZ, SZ)).real
-        super().__init__(size, interaction=Hint)
+        coupling = (0.25 * Jxy * (np.kron(SX, SX) + np.kron(SY, SY))
+                    + 0.25 * Jz * np.kron(SZ, SZ)).real
+        super().__init__(size, interaction=coupling)

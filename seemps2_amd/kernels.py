@@ -33,6 +33,25 @@ def as_dev(x) -> torch.Tensor:
     return t.to(_lib.device(), non_blocking=False)
 
 
+def need(*tensors: torch.Tensor | None) -> None:
+    """Raise ValueError unless every tensor (None is skipped) is a contiguous
+    complex128 tensor on a HIP device: the kernels read raw device pointers
+    with row-major complex128 strides, so anything else must be converted
+    (:func:`as_dev`) before a launch."""
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"expected a device tensor, got {type(t).__name__}")
+        if t.device.type != "cuda":
+            raise ValueError("expected a tensor on the HIP device, got one on "
+                             f"{t.device.type}")
+        if t.dtype != CDTYPE:
+            raise ValueError(f"expected a complex128 tensor, got {t.dtype}")
+        if not t.is_contiguous():
+            raise ValueError("expected a contiguous tensor")
+
+
 def to_numpy(x: torch.Tensor) -> np.ndarray:
     return x.detach().cpu().numpy()
 
@@ -59,6 +78,7 @@ def gemm(A: torch.Tensor, B: torch.Tensor, opA: int = 0, opB: int = 0, alpha=1.0
          beta=0.0, C: torch.Tensor | None = None, precision: str = "fp64") -> torch.Tensor:
     """C = alpha op(A) op(B) + beta C for 2-D contiguous complex128 tensors
     (``precision`` "fp32": FP32 products and accumulation)."""
+    need(A, B, C)
     if A.dim() != 2 or B.dim() != 2:
         raise ValueError("gemm expects matrices")
     M = A.shape[1] if opA in (1, 2) else A.shape[0]
@@ -89,6 +109,9 @@ def gemm(A: torch.Tensor, B: torch.Tensor, opA: int = 0, opB: int = 0, alpha=1.0
 def local(x: torch.Tensor, Mop: torch.Tensor, P: int, I: int, Q: int, J1: int, J2: int,
           out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
     """out[p, j1, q, j2] (+)= sum_i x[p, i, q] Mop[i, j1, j2]."""
+    need(x, Mop, out)
+    if x.numel() != P * I * Q or Mop.numel() != I * J1 * J2:
+        raise ValueError("local: operand sizes do not match the dimensions")
     if out is None:
         out = empty(P, J1, Q, J2)
     check(lib.seemps_zlocal(P, I, Q, J1, J2, ptr(x), ptr(Mop), ptr(out), int(accumulate),
@@ -98,6 +121,9 @@ def local(x: torch.Tensor, Mop: torch.Tensor, P: int, I: int, Q: int, J1: int, J
 
 def axpby(a, x: torch.Tensor | None, b, y: torch.Tensor, conj_x: bool = False) -> torch.Tensor:
     """y = a op(x) + b y in place."""
+    need(x, y)
+    if x is not None and x.numel() != y.numel():
+        raise ValueError("axpby: sizes differ")
     a, b = complex(a), complex(b)
     check(lib.seemps_zaxpby(y.numel(), a.real, a.imag, ptr(x), int(conj_x), b.real, b.imag,
                             ptr(y), stream()))
@@ -140,6 +166,7 @@ def embed_block(B: torch.Tensor, A: torch.Tensor, l_off: int, r_off: int) -> Non
 
 def site_product(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """out[(x,y), k, (u,v)] = A[x,k,u] B[y,k,v] for MPS tensors A (a,d,b), B (c,d,e)."""
+    need(A, B)
     a, d, b = A.shape
     c, d2, e = B.shape
     if d2 != d:
@@ -151,6 +178,7 @@ def site_product(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
 
 def dotc_dev(x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
     """Device scalar sum(conj(x) * y)."""
+    need(x, y)
     n = x.numel()
     if y.numel() != n:
         raise ValueError("dotc: sizes differ")
@@ -176,6 +204,7 @@ def truncate(s: torch.Tensor, method: int, tol: float, max_bond: int) -> tuple[i
 
 def svd(A: torch.Tensor):
     """Economy SVD of a device matrix: (U, S, VH), S descending FP64."""
+    need(A)
     m, n = A.shape
     r = min(m, n)
     U = empty(m, r)
@@ -190,6 +219,9 @@ def svd(A: torch.Tensor):
 def svd_split(AA: torch.Tensor, m: int, n: int, mode: int, method: int, tol: float,
               max_bond: int, want_s: bool = False):
     """SVD + truncation + split; returns (A (m x D), B (D x n), D, err^2, s or None)."""
+    need(AA)
+    if AA.numel() != m * n:
+        raise ValueError("svd_split: matrix size does not match m x n")
     r = min(m, n)
     outA = empty(m * r)
     outB = empty(r * n)
@@ -262,6 +294,7 @@ def svd_split_basis(AA: torch.Tensor, m: int, n: int, side: int, basis: torch.Te
 
 
 def permute(x: torch.Tensor, perm) -> torch.Tensor:
+    need(x)
     dims = (ctypes.c_int64 * x.dim())(*x.shape)
     p = (ctypes.c_int * x.dim())(*perm)
     out = empty(*[x.shape[k] for k in perm])
@@ -269,8 +302,28 @@ def permute(x: torch.Tensor, perm) -> torch.Tensor:
     return out
 
 
+def permuted(x: torch.Tensor, perm) -> torch.Tensor:
+    """``x.permute(perm)`` materialised once per tensor.
+
+    The copy is kept in a dictionary attached to ``x`` itself, so it is freed
+    with ``x``, and it is recomputed when ``x`` has been modified in place
+    (torch's version counter of ``x`` moved)."""
+    key = tuple(perm)
+    cache = getattr(x, "_seemps_permuted", None)
+    if cache is None:
+        cache = {}
+        x._seemps_permuted = cache
+    hit = cache.get(key)
+    if hit is not None and hit[0] == x._version:
+        return hit[1]
+    y = permute(x, perm)
+    cache[key] = (x._version, y)
+    return y
+
+
 def two_site_operator(O1: torch.Tensor, O2: torch.Tensor) -> torch.Tensor:
     """Hm[(c,k,l),(ko,lo,d)] from MPO tensors O1[c,ko,k,e], O2[e,lo,l,d]."""
+    need(O1, O2)
     c, d1, d1b, e = O1.shape
     e2, d2, d2b, w2 = O2.shape
     if d1 != d1b or d2 != d2b or e != e2:
@@ -282,6 +335,7 @@ def two_site_operator(O1: torch.Tensor, O2: torch.Tensor) -> torch.Tensor:
 
 def two_site_matvec(L: torch.Tensor, Hm: torch.Tensor, R: torch.Tensor, v: torch.Tensor,
                     d1: int, d2: int) -> torch.Tensor:
+    need(L, Hm, R, v)
     a, c, b = L.shape
     g, w2, f = R.shape
     out = empty(a, d1, d2, g)
@@ -300,6 +354,7 @@ def eigensolve(L: torch.Tensor, Hm: torch.Tensor, R: torch.Tensor, v0: torch.Ten
     ``precision`` selects the arithmetic of the operator's contractions: "fp64",
     "fp32", or "mixed" (FP32 restarts down to a 1e-5 relative residual, FP64 to
     ``tol``)."""
+    need(L, Hm, R, v0)
     chiL, c, _ = L.shape
     chiR, w2, _ = R.shape
     evec = empty(chiL, d1, d2, chiR)

@@ -21,17 +21,9 @@ from ..state.environments import (
 )
 from ..state.mps import TensorArray
 
-_APPLY_OPS: dict = {}
-
-
 def _apply_op(W):
-    key = (W.data_ptr(), tuple(W.shape))
-    M = _APPLY_OPS.get(key)
-    if M is None or M[0] is not W:
-        # Mop[j, (a, i), b] = W[a, i, j, b]
-        M = (W, K.permute(W, (2, 0, 1, 3)))
-        _APPLY_OPS[key] = M
-    return M[1]
+    # Mop[j, (a, i), b] = W[a, i, j, b], materialised once per MPO tensor
+    return K.permuted(W, (2, 0, 1, 3))
 
 
 def _mpo_multiply_tensor(A, B):

@@ -22,29 +22,15 @@ from ..core import (
     vdot,
 )
 
-# Per-MPO-tensor cache of the permuted small operators (keyed by data_ptr).
-_LEFT_OPS: dict = {}
-_RIGHT_OPS: dict = {}
-
 
 def _left_op(O):
-    key = (O.data_ptr(), tuple(O.shape))
-    M = _LEFT_OPS.get(key)
-    if M is None or M[0] is not O:
-        # Mop[(c,i), (j,e)] = O[c,j,i,e]
-        M = (O, K.permute(O, (0, 2, 1, 3)))
-        _LEFT_OPS[key] = M
-    return M[1]
+    # Mop[(c,i), (j,e)] = O[c,j,i,e], materialised once per MPO tensor
+    return K.permuted(O, (0, 2, 1, 3))
 
 
 def _right_op(O):
-    key = (O.data_ptr(), tuple(O.shape))
-    M = _RIGHT_OPS.get(key)
-    if M is None or M[0] is not O:
-        # Mop[(j,e), (c,i)] = O[c,j,i,e]
-        M = (O, K.permute(O, (1, 3, 0, 2)))
-        _RIGHT_OPS[key] = M
-    return M[1]
+    # Mop[(j,e), (c,i)] = O[c,j,i,e], materialised once per MPO tensor
+    return K.permuted(O, (1, 3, 0, 2))
 
 
 def begin_mpo_environment():
@@ -53,6 +39,7 @@ def begin_mpo_environment():
 
 def update_left_mpo_environment(rho, A, O, B):
     """L'[d,e,f] = sum rho[a,c,b] A*[a,j,d] O[c,j,i,e] B[b,i,f] (environments.py:19-30)."""
+    rho, A, O, B = K.as_dev(rho), K.as_dev(A), K.as_dev(O), K.as_dev(B)
     a, c, b = rho.shape
     _, j, d = A.shape
     c2, jo, i, e = O.shape
@@ -64,6 +51,7 @@ def update_left_mpo_environment(rho, A, O, B):
 
 def update_right_mpo_environment(rho, A, O, B):
     """R'[a,c,b] = sum rho[d,e,f] A*[a,j,d] O[c,j,i,e] B[b,i,f] (environments.py:33-42)."""
+    rho, A, O, B = K.as_dev(rho), K.as_dev(A), K.as_dev(O), K.as_dev(B)
     d, e, f = rho.shape
     a, j, _ = A.shape
     c, jo, i, e2 = O.shape

@@ -1,0 +1,350 @@
+"""The five benchmark workloads at their stated sizes, on the device path.
+
+Each test runs one workload at full size and checks it against the oracle
+(the NumPy/SciPy restatement of the reference algorithm) on the same seeded
+inputs, against a closed form, or, where neither fits in a test, through
+size-independent properties (bond dimensions decided by the same rule,
+truncation errors bounded by the full SVD's).  The achieved differences are
+printed so that a run with ``-s`` shows them.
+
+Tolerances are relative unless stated.  Where a bar is looser than 1e-12 the
+test says why (amplification of rounding by a small spectral gap at the
+truncation cut, or the stopping tolerance of an iterative solver).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import core as oc, algorithms as oa, builders as ob, exact as ex
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def S():
+    import seemps2_amd
+
+    return seemps2_amd
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def fidelity(a, b):
+    return float(abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real))
+
+
+def report(name, **values):
+    print(f"[{name}] " + ", ".join(f"{k}={v}" for k, v in values.items()), flush=True)
+
+
+def nn_apply_dense(h, v, N, d=2):
+    """sum_i 1 (x) h_{i,i+1} (x) 1 applied to a dense state vector."""
+    out = np.zeros_like(v)
+    for i in range(N - 1):
+        x = v.reshape(d ** i, d * d, d ** (N - i - 2))
+        out += np.einsum("ab,xbz->xaz", h, x).reshape(-1)
+    return out
+
+
+# --------------------------------------------------------------- workload 1
+
+
+def test_config1_apply_simplify_n20_chi32_to_64(S):
+    """Random complex MPS, N = 20, d = 2, uniform chi = 32, normalised; the
+    Heisenberg MPO (bond 5) applied and simplified with tolerance 1e-10, max
+    bond 64, 2 sweeps.  Against the oracle on the same inputs: identical bond
+    dimensions, the same state and error; against the dense H psi (2^20)."""
+    rng = np.random.default_rng(1001)
+    N, D = 20, 32
+    a = ob.random_mps([2] * N, D=D, truncate=False, complex=True, rng=rng)
+    a.tensors[0] = a.tensors[0] / math.sqrt(a.norm_squared())
+    W = ob.heisenberg_mpo(N)
+    kw = dict(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=64,
+              max_sweeps=2)
+    ref = oa.mpo_apply(W, a, oc.Strategy(**kw), simplify_flag=True)
+    out = S.MPO(W).apply(S.MPS(a.tensors), strategy=S.Strategy(**kw), simplify=True)
+    assert out.bond_dimensions() == ref.bond_dimensions()
+    assert max(out.bond_dimensions()) == 64
+    target = nn_apply_dense(ob.heisenberg_interaction(), a.to_vector(), N)
+    v_out, v_ref = out.to_vector(), ref.to_vector()
+    f_out, f_ref = fidelity(v_out, target), fidelity(v_ref, target)
+    d_state = rel(v_out, v_ref)
+    d_err = abs(out.error() - ref.error) / ref.error
+    report("config1", fidelity=f_out, fidelity_oracle=f_ref, state_rel_diff=d_state,
+           error=out.error(), error_rel_diff=d_err)
+    # The truncation to 64 cuts a random state's flat spectrum: rounding
+    # differences in the SVDs move the kept subspace by (rounding / gap at the
+    # cut), so states agree to ~1e-11 rather than 1e-12; fidelity with the
+    # exact H psi is a smooth function of the state and agrees to 1e-12.
+    assert abs(f_out - f_ref) < 1e-12
+    assert d_state < 1e-10
+    assert d_err < 1e-10
+
+
+# --------------------------------------------------------------- workload 2
+
+
+@pytest.fixture(scope="module")
+def xxz_chi256(S):
+    """XXZ chain (Jxy = 1, Jz = 0.5), N = 64, from a seeded random chi = 16 MPS,
+    eight two-site DMRG sweeps grow the bonds to 256; the state is then centred
+    on bond (31, 32)."""
+    N, chi, site = 64, 256, 31
+    rng = np.random.default_rng(2024)
+    H = S.XXZHamiltonian(N, 1.0, 0.5).to_mpo()
+    guess = ob.random_mps([2] * N, D=16, complex=True, rng=rng)
+    st = S.DEFAULT_STRATEGY.replace(max_bond_dimension=chi)
+    res = S.dmrg(H, S.MPS(guess.tensors), maxiter=8, strategy=st)
+    psi = res.state
+    psi.recenter(site)
+    assert psi.bond_dimensions()[site + 1] == chi
+    tensors = [A.cpu().numpy() for A in psi]
+    W = [O.cpu().numpy() for O in H]
+    return dict(H=H, psi=psi, tensors=tensors, W=W, site=site, chi=chi, strategy=st,
+                energy=res.energy)
+
+
+def _oracle_bond(xxz, tol):
+    o = oa.OMPS([T.copy() for T in xxz["tensors"]], 0.0, xxz["site"])
+    QF = oa.QuadraticForm(xxz["W"], o, start=xxz["site"])
+    return QF, oa.diagonalize_two_site(QF, xxz["site"], tol)
+
+
+def test_config2_bond_update_plain_vs_oracle(S, xxz_chi256):
+    """One chi x chi bond update of the plain FP64 path from the chi = 256 DMRG
+    state, against the oracle from the same tensors: eigenvalue to 1e-12, and
+    the split of the same two-site tensor with the same bond dimension,
+    spectrum and truncation error."""
+    import importlib
+
+    from seemps2_amd import kernels as K
+
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
+    x = xxz_chi256
+    i, chi = x["site"], x["chi"]
+    QF = S.QuadraticForm(x["H"], x["psi"], start=i)
+    E_gpu, AB = D._diagonalize_two_site(QF, i, 1e-12)
+    _, (E_ref, AB_ref) = _oracle_bond(x, 1e-12)
+    dE = abs(E_gpu - E_ref) / abs(E_ref)
+    # the eigenvectors agree up to a phase
+    v, w = AB.cpu().numpy().reshape(-1), AB_ref.reshape(-1)
+    ov = 1 - abs(np.vdot(v, w)) / (np.linalg.norm(v) * np.linalg.norm(w))
+    # the same theta through both splits
+    a, d1, d2, b = AB_ref.shape
+    st = x["strategy"]
+    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=chi, normalize=True)
+    A2, B2, Dk, err2, s = K.svd_split(K.as_dev(AB_ref.reshape(a * d1, d2 * b)), a * d1, d2 * b,
+                                      0, st.method, st.tolerance, chi, want_s=True)
+    U_ref, sV_ref, err_ref = oc.left_orth_2site(AB_ref.copy(), ost)
+    s_ref = np.linalg.svd(AB_ref.reshape(a * d1, d2 * b), compute_uv=False)
+    rec = (A2 @ B2).cpu().numpy()
+    rec_ref = U_ref.reshape(a * d1, -1) @ sV_ref.reshape(-1, d2 * b)
+    d_s = np.abs(s.cpu().numpy() - s_ref[:Dk]).max() / s_ref[0]
+    d_rec = rel(rec, rec_ref)
+    d_err = abs(math.sqrt(err2) - err_ref) / max(err_ref, 1e-300)
+    report("config2 plain", E=E_gpu, E_rel_diff=dE, evec_1_minus_overlap=ov, D=Dk,
+           D_ref=U_ref.shape[2], sigma_max_diff=d_s, reconstruction_rel_diff=d_rec,
+           sqrt_err=math.sqrt(err2), sqrt_err_rel_diff=d_err)
+    assert dE < 1e-12
+    assert ov < 1e-9  # eigenvectors: Lanczos stopping tolerance 1e-12 on the residual
+    assert Dk == U_ref.shape[2] == chi
+    assert d_s < 1e-13
+    assert d_rec < 1e-12
+    assert d_err < 1e-9  # sqrt of a sum of ~1e-10 tail weights: cancellation-free to 1e-9
+
+
+def test_config2_bond_update_chosen_form_vs_oracle(S, xxz_chi256):
+    """The same bond update in the chosen form (FP32-then-FP64 Lanczos and the
+    basis-sketched learned split): eigenvalue and kept spectrum within 1e-9 of
+    the oracle's, same bond dimension."""
+    import importlib
+
+    from seemps2_amd import learned
+
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
+    x = xxz_chi256
+    i, chi = x["site"], x["chi"]
+    psi = S.CanonicalMPS([A.clone() for A in x["psi"]], center=i, is_canonical=True)
+    QF = S.QuadraticForm(x["H"], psi, start=i)
+    E_gpu, AB = D._diagonalize_two_site(QF, i, 1e-10, precision="mixed")
+    _, (E_ref, AB_ref) = _oracle_bond(x, 1e-10)
+    net = learned.RankPredictor(seed=1)
+    lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, min_size=64)
+    with learned.learned_truncation(lt):
+        QF.update_2site_right(AB, i, x["strategy"])
+    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=chi, normalize=True)
+    U_ref, sV_ref, err_ref = oc.left_orth_2site(AB_ref.copy(), ost)
+    new_A = psi[i].cpu().numpy()
+    dE = abs(E_gpu - E_ref) / abs(E_ref)
+    # kept subspaces: projector difference of the left factors
+    Pa = new_A.reshape(-1, new_A.shape[2])
+    Pr = U_ref.reshape(-1, U_ref.shape[2])
+    d_sub = np.linalg.norm(Pa @ (Pa.conj().T @ Pr) - Pr) / math.sqrt(Pr.shape[1])
+    report("config2 chosen", E=E_gpu, E_rel_diff=dE, D=new_A.shape[2], D_ref=Pr.shape[1],
+           basis_splits=lt.basis_splits, fallbacks=lt.fallbacks, subspace_diff=d_sub,
+           error=psi.error())
+    assert dE < 1e-9
+    assert new_A.shape[2] == Pr.shape[1] == chi
+    assert lt.calls == 1
+    assert d_sub < 1e-4  # vectors near the cut rotate with sigma_256 / sigma_257 gaps
+
+
+def test_config2_xx_chain_free_fermions_n64_chi256(S):
+    """The XX chain (Jz = 0) at N = 64 with bond cap 256 against the free-fermion
+    closed form."""
+    N = 64
+    rng = np.random.default_rng(64)
+    g = ob.random_mps([2] * N, D=16, complex=True, rng=rng)
+    st = S.DEFAULT_STRATEGY.replace(max_bond_dimension=256)
+    out = S.dmrg(S.XXZHamiltonian(N, 1.0, 0.0), S.MPS(g.tensors), maxiter=10, strategy=st)
+    E0 = ex.xx_free_fermion_energy(N)
+    d = abs(out.energy - E0) / abs(E0)
+    report("config2 XX", E=out.energy, E_exact=E0, rel_diff=d, sweeps=len(out.trajectory) - 1,
+           max_bond=out.state.max_bond_dimension(), error=out.state.error())
+    assert out.energy >= E0 - 1e-12 * abs(E0)  # variational
+    assert d < 1e-10
+
+
+# --------------------------------------------------------------- workload 3
+
+
+def _qtt_problem(S, n, tol):
+    I = S.QuantizedInterval(-1.0, 1.0, n)
+    x = I.to_vector()
+    fx = np.exp(-x ** 2 / 0.1) * np.cos(20 * x)
+    fy = np.exp(-x ** 2 / 0.1)
+    st = S.Strategy(tolerance=tol, simplification_tolerance=tol, max_bond_dimension=128)
+    psi = S.mps_tensor_product([S.MPS.from_vector(fx, [2] * n, normalize=False),
+                                S.MPS.from_vector(fy, [2] * n, normalize=False)], "B")
+    D1 = S.finite_differences_mpo(2, I)
+    Lx = D1.extend(2 * n, sites=[2 * k for k in range(n)])
+    Ly = D1.extend(2 * n, sites=[2 * k + 1 for k in range(n)])
+    Lap = (Lx + Ly).join(strategy=st)
+    perm = S.interleaving_permutation([n, n])
+    d1 = sp.csr_matrix(np.real_if_close(D1.to_matrix()))
+    return psi, Lap, st, d1, np.outer(fx, fy), perm
+
+
+def _grid(v, n, perm):
+    """MPS vector in "B" site order -> f[x, y]."""
+    return v.reshape([2] * (2 * n)).transpose(np.argsort(perm)).reshape(2 ** n, 2 ** n)
+
+
+def test_config3_qtt_laplacian_euler_n24(S):
+    """f(x, y) = exp(-(x^2 + y^2)/0.1) cos(20 x) on 2^12 x 2^12 points, 24
+    interleaved qubits, the 2-D finite-difference Laplacian (bond ~5), explicit
+    Euler with dt = 1e-6 and simplify to tolerance 1e-9, max bond 128.
+
+    With grid step h = 2/4096 the Laplacian's largest eigenvalue is
+    ~8/h^2 = 3.4e7, so dt = 1e-6 is 17x beyond explicit Euler's stability
+    limit (dt < 2 h^2 / 8): the exact iteration multiplies its highest modes by
+    ~33 per step and its norm grows by ~1e60 over 50 steps.  The test therefore
+    checks the stated workload where it is meaningful: the first steps against
+    the dense exact iteration and against the oracle's MPO.apply + simplify at
+    full size, and the 50 steps of the same problem at a stable dt = 1e-8
+    against the dense exact iteration."""
+    n = 12
+    psi0, Lap, st, d1, F0, perm = _qtt_problem(S, n, 1e-9)
+    assert max(W.shape[0] for W in Lap) <= 6
+
+    def exact(F, dt, steps):
+        for _ in range(steps):
+            F = F + dt * (d1 @ F + (d1 @ F.T).T)
+        return F
+
+    assert rel(_grid(psi0.to_vector(), n, perm), F0) < 1e-13
+    # stated time step: 3 steps against the dense iteration and the oracle
+    dt = 1e-6
+    out = S.euler(Lap, (0.0, 3 * dt), psi0, steps=3, strategy=st)
+    F3 = exact(F0, dt, 3)
+    e3 = rel(_grid(out.to_vector(), n, perm), F3)
+    ost = oc.Strategy(tolerance=1e-9, simplification_tolerance=1e-9, max_bond_dimension=128)
+    W = [A.cpu().numpy() for A in Lap]
+    o = oa.OMPS([A.cpu().numpy() for A in psi0], 0.0)
+    mine = psi0
+    d_or = []
+    for _ in range(2):
+        Lo = oa.mpo_apply(W, o, ost, simplify_flag=True)
+        o = oa.simplify(oa.OMPSSum([1.0, dt], [o, Lo]), ost)
+        mine = S.euler(Lap, (0.0, dt), mine, steps=1, strategy=st)
+        assert mine.bond_dimensions() == o.bond_dimensions()
+        d_or.append(rel(mine.to_vector(), o.to_vector()))
+    # stated workload to the end (reported: the exact iteration diverges too)
+    out50 = S.euler(Lap, (0.0, 50 * dt), psi0, steps=50, strategy=st)
+    F50 = exact(F0, dt, 50)
+    n50 = np.linalg.norm(out50.to_vector()) / np.linalg.norm(F50)
+    # stable time step: 50 steps against the dense iteration
+    dts = 1e-8
+    outs = S.euler(Lap, (0.0, 50 * dts), psi0, steps=50, strategy=st)
+    Fs = exact(F0, dts, 50)
+    es = rel(_grid(outs.to_vector(), n, perm), Fs)
+    report("config3", err_3_steps=e3, oracle_rel_diff=d_or, norm_ratio_50_steps=n50,
+           exact_norm_growth=np.linalg.norm(F50) / np.linalg.norm(F0), err_stable_50=es,
+           max_bond=outs.max_bond_dimension(), error_bound=outs.error() / np.linalg.norm(Fs))
+    assert e3 < 1e-6
+    assert max(d_or) < 1e-9
+    assert np.isfinite(n50)
+    # simplify to 1e-9 (relative norm-squared) per step: the L2 error stays
+    # within the accumulated truncation bound
+    assert es <= 1.1 * outs.error() / np.linalg.norm(Fs) + 1e-10
+
+
+# --------------------------------------------------------------- workload 5
+
+
+def _haar(rng, n):
+    Q, R = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    return Q * (np.diag(R) / np.abs(np.diag(R)))
+
+
+def test_config5_haar_layers_learned_vs_full_n48_d4_chi512(S):
+    """N = 48, d = 4, product state, 6 layers of seeded Haar 16 x 16 two-site
+    unitaries (each layer one sweep over the 47 bonds, as the reference's
+    pairwise-unitary layers), bond cap 512, tolerance 1e-8: the learned-rank
+    splits against full SVD splits.  Bond dimensions must be decided the same
+    way, the truncation errors agree, the final states agree, and the fallback
+    rate is reported."""
+    from seemps2_amd import learned
+    from seemps2_amd.evolution.trotter import _sweep_gates
+
+    rng = np.random.default_rng(48)
+    N, d, layers = 48, 4, 6
+    gates = [[_haar(rng, d * d) for _ in range(N - 1)] for _ in range(layers)]
+    st = S.Strategy(tolerance=1e-8, max_bond_dimension=512)
+
+    def run():
+        psi = S.CanonicalMPS(S.product_state(np.eye(d)[0], N), center=0, strategy=st)
+        for layer in gates:
+            psi = _sweep_gates(psi, layer, st)
+        torch.cuda.synchronize()
+        return psi
+
+    plain = run()
+    # the predictor learns from the plain path's own spectra
+    log = learned.SpectrumLog()
+    with learned.recording(log):
+        run()
+    X, t = log.tensors()
+    net = learned.RankPredictor(seed=5)
+    net.train(X, t, epochs=300)
+    lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, min_size=64)
+    with learned.learned_truncation(lt):
+        out = run()
+    assert out.bond_dimensions() == plain.bond_dimensions()
+    assert plain.max_bond_dimension() == 512
+    f = abs(complex(S.scprod(out, plain))) / math.sqrt(
+        abs(complex(S.scprod(out, out))) * abs(complex(S.scprod(plain, plain))))
+    d_err = abs(out.error() - plain.error()) / plain.error()
+    sketched = lt.calls - lt.full_splits
+    report("config5", bonds=max(plain.bond_dimensions()), error_full=plain.error(),
+           error_learned=out.error(), error_rel_diff=d_err, fidelity=f,
+           splits=lt.calls, sketched=sketched, fallbacks=lt.fallbacks,
+           fallback_rate=lt.fallbacks / max(sketched, 1))
+    assert 1 - f < 1e-9
+    assert d_err < 1e-9
