@@ -176,6 +176,15 @@ int64_t seemps_reduce_workspace(int64_t n);
 int seemps_zdotc(int64_t n, const void* x, const void* y, void* result,
                  void* work, int64_t work_bytes, void* stream);
 
+/* seemps_zdotc without the workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums go to a library-owned static device array of 512 complex
+ * values (8 KiB per GPU, held until the process ends), enough for every n.
+ * Allocates nothing.  Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or events). */
+int seemps_zdotc_noalloc(int64_t n, const void* x, const void* y, void* result, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Truncation and SVD
  * ------------------------------------------------------------------------- */
