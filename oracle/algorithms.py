@@ -20,8 +20,11 @@ Pinning (tests/test_oracle_algorithms.py): dense brute force (state vectors,
 operator matrices) for MPO.apply / expectation / environments / the two-site
 matvec; exactness of simplify without truncation, dense fidelity with
 truncation; exact diagonalisation (N <= 10) and free fermions (XX chain) for
-dmrg.  The sweep-by-sweep simplify error and the DMRG trajectory before
-convergence depend on gauge / ARPACK iterates: parity unpinned for those
+dmrg.  simplify's per-sweep error measure, its stopping sweep and the
+accumulated error bound are pinned sweep by sweep against dense vectors, with
+planted bookkeeping mistakes that the check catches
+(tests/test_oracle_simplify_bookkeeping.py).  The DMRG trajectory before
+convergence depends on ARPACK iterates: parity unpinned for those
 intermediate values (final converged quantities are pinned).
 """
 
@@ -231,10 +234,14 @@ def zero_state(state: OMPS) -> OMPS:
     return OMPS([np.zeros((1, A.shape[1], 1)) for A in state.tensors], 0.0, 0)
 
 
-def simplify(state, strategy: Strategy, direction: int = +1, guess=None):
-    """``simplification.py:32-144`` and ``:150-280`` (MPSSum)."""
+def simplify(state, strategy: Strategy, direction: int = +1, guess=None, observer=None):
+    """``simplification.py:32-144`` and ``:150-280`` (MPSSum).
+
+    ``observer`` (test hook, not in the reference): called after every sweep
+    as observer(tensors, err, last) with a copy of the current tensors, the
+    sweep's error measure and the site that holds the centre."""
     if isinstance(state, OMPSSum):
-        return _simplify_sum(state, strategy, direction, guess)
+        return _simplify_sum(state, strategy, direction, guess, observer)
     size = state.size
     start = 0 if direction > 0 else size - 1
     method = strategy.simplify
@@ -248,11 +255,11 @@ def simplify(state, strategy: Strategy, direction: int = +1, guess=None):
     if not norm_state_sqr:
         return zero_state(state)
     form = AntilinearForm(mps, state, center=start)
-    sweeps, err = _sweeps(mps, [1.0], [form], strategy, direction, norm_state_sqr)
+    sweeps, err = _sweeps(mps, [1.0], [form], strategy, direction, norm_state_sqr, observer)
     return _finish(mps, state.error + sqrt(err), strategy, direction, sweeps, err)
 
 
-def _simplify_sum(s: OMPSSum, strategy, direction, guess):
+def _simplify_sum(s: OMPSSum, strategy, direction, guess, observer=None):
     norm_state_sqr = delete_zero_components(s)
     if not norm_state_sqr:
         return zero_state(s.states[0])
@@ -266,11 +273,11 @@ def _simplify_sum(s: OMPSSum, strategy, direction, guess):
         return canonical(join(s), size - 1 - start, strategy)
     mps = canonical(join(s) if guess is None else guess, start, strategy, normalize=False)
     forms = [AntilinearForm(mps, si, center=start) for si in s.states]
-    sweeps, err = _sweeps(mps, s.weights, forms, strategy, direction, norm_state_sqr)
+    sweeps, err = _sweeps(mps, s.weights, forms, strategy, direction, norm_state_sqr, observer)
     return _finish(mps, s.error() + sqrt(err), strategy, direction, sweeps, err)
 
 
-def _sweeps(mps, weights, forms, strategy, direction, norm_state_sqr):
+def _sweeps(mps, weights, forms, strategy, direction, norm_state_sqr, observer=None):
     size = mps.size
     err = 2.0
     history = []
@@ -295,6 +302,8 @@ def _sweeps(mps, weights, forms, strategy, direction, norm_state_sqr):
         old_err = err
         err = 2 * abs(1.0 - sc.real / sqrt(norm_mps_sqr * norm_state_sqr))
         history.append(err)
+        if observer is not None:
+            observer([A.copy() for A in mps.tensors], err, last)
         if err < strategy.simplification_tolerance or err > old_err:
             break
         direction = -direction

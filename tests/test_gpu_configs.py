@@ -92,21 +92,26 @@ def test_config1_apply_simplify_n20_chi32_to_64(S):
 
 @pytest.fixture(scope="module")
 def xxz_chi256(S):
-    """XXZ chain (Jxy = 1, Jz = 0.5), N = 64, from a seeded random chi = 16 MPS,
-    eight two-site DMRG sweeps grow the bonds to 256; the state is then centred
-    on bond (31, 32)."""
-    N, chi, site = 64, 256, 31
+    """The workload's DMRG: XXZ chain (Jxy = 1, Jz = 0.5), N = 64, seeded random
+    chi = 16 MPS, dmrg(maxiter=10) with bond cap 256 and the default tolerance.
+    The state is then centred on the bond in the middle of the chain (31, 32);
+    its size is what the Strategy rule keeps there (at most the cap)."""
+    N, cap, site = 64, 256, 31
     rng = np.random.default_rng(2024)
     H = S.XXZHamiltonian(N, 1.0, 0.5).to_mpo()
     guess = ob.random_mps([2] * N, D=16, complex=True, rng=rng)
-    st = S.DEFAULT_STRATEGY.replace(max_bond_dimension=chi)
-    res = S.dmrg(H, S.MPS(guess.tensors), maxiter=8, strategy=st)
+    st = S.DEFAULT_STRATEGY.replace(max_bond_dimension=cap)
+    res = S.dmrg(H, S.MPS(guess.tensors), maxiter=10, strategy=st)
     psi = res.state
     psi.recenter(site)
-    assert psi.bond_dimensions()[site + 1] == chi
+    chi = psi.bond_dimensions()[site + 1]
+    report("config2 state", energy=res.energy, sweeps=len(res.trajectory) - 1,
+           bonds_at_cap=sum(b == cap for b in psi.bond_dimensions()), middle_bond=chi,
+           max_bond=psi.max_bond_dimension())
+    assert chi >= 192
     tensors = [A.cpu().numpy() for A in psi]
     W = [O.cpu().numpy() for O in H]
-    return dict(H=H, psi=psi, tensors=tensors, W=W, site=site, chi=chi, strategy=st,
+    return dict(H=H, psi=psi, tensors=tensors, W=W, site=site, chi=chi, cap=cap, strategy=st,
                 energy=res.energy)
 
 
@@ -117,9 +122,9 @@ def _oracle_bond(xxz, tol):
 
 
 def test_config2_bond_update_plain_vs_oracle(S, xxz_chi256):
-    """One chi x chi bond update of the plain FP64 path from the chi = 256 DMRG
-    state, against the oracle from the same tensors: eigenvalue to 1e-12, and
-    the split of the same two-site tensor with the same bond dimension,
+    """One bond update of the plain FP64 path in the middle of the workload's
+    DMRG state, against the oracle from the same tensors: eigenvalue to 1e-12,
+    and the split of the same two-site tensor with the same bond dimension,
     spectrum and truncation error."""
     import importlib
 
@@ -127,7 +132,7 @@ def test_config2_bond_update_plain_vs_oracle(S, xxz_chi256):
 
     D = importlib.import_module("seemps2_amd.optimization.dmrg")
     x = xxz_chi256
-    i, chi = x["site"], x["chi"]
+    i, cap = x["site"], x["cap"]
     QF = S.QuadraticForm(x["H"], x["psi"], start=i)
     E_gpu, AB = D._diagonalize_two_site(QF, i, 1e-12)
     _, (E_ref, AB_ref) = _oracle_bond(x, 1e-12)
@@ -138,9 +143,9 @@ def test_config2_bond_update_plain_vs_oracle(S, xxz_chi256):
     # the same theta through both splits
     a, d1, d2, b = AB_ref.shape
     st = x["strategy"]
-    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=chi, normalize=True)
+    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=cap, normalize=True)
     A2, B2, Dk, err2, s = K.svd_split(K.as_dev(AB_ref.reshape(a * d1, d2 * b)), a * d1, d2 * b,
-                                      0, st.method, st.tolerance, chi, want_s=True)
+                                      0, st.method, st.tolerance, cap, want_s=True)
     U_ref, sV_ref, err_ref = oc.left_orth_2site(AB_ref.copy(), ost)
     s_ref = np.linalg.svd(AB_ref.reshape(a * d1, d2 * b), compute_uv=False)
     rec = (A2 @ B2).cpu().numpy()
@@ -153,7 +158,7 @@ def test_config2_bond_update_plain_vs_oracle(S, xxz_chi256):
            sqrt_err=math.sqrt(err2), sqrt_err_rel_diff=d_err)
     assert dE < 1e-12
     assert ov < 1e-9  # eigenvectors: Lanczos stopping tolerance 1e-12 on the residual
-    assert Dk == U_ref.shape[2] == chi
+    assert Dk == U_ref.shape[2]
     assert d_s < 1e-13
     assert d_rec < 1e-12
     assert d_err < 1e-9  # sqrt of a sum of ~1e-10 tail weights: cancellation-free to 1e-9
@@ -169,7 +174,7 @@ def test_config2_bond_update_chosen_form_vs_oracle(S, xxz_chi256):
 
     D = importlib.import_module("seemps2_amd.optimization.dmrg")
     x = xxz_chi256
-    i, chi = x["site"], x["chi"]
+    i, cap = x["site"], x["cap"]
     psi = S.CanonicalMPS([A.clone() for A in x["psi"]], center=i, is_canonical=True)
     QF = S.QuadraticForm(x["H"], psi, start=i)
     E_gpu, AB = D._diagonalize_two_site(QF, i, 1e-10, precision="mixed")
@@ -178,21 +183,32 @@ def test_config2_bond_update_chosen_form_vs_oracle(S, xxz_chi256):
     lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, min_size=64)
     with learned.learned_truncation(lt):
         QF.update_2site_right(AB, i, x["strategy"])
-    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=chi, normalize=True)
+    ost = oc.DEFAULT_STRATEGY.replace(max_bond_dimension=cap, normalize=True)
     U_ref, sV_ref, err_ref = oc.left_orth_2site(AB_ref.copy(), ost)
-    new_A = psi[i].cpu().numpy()
+    a, d1, d2, b = AB_ref.shape
+    new_A, new_B = psi[i].cpu().numpy(), psi[i + 1].cpu().numpy()
+    D_new = new_A.shape[2]
     dE = abs(E_gpu - E_ref) / abs(E_ref)
-    # kept subspaces: projector difference of the left factors
-    Pa = new_A.reshape(-1, new_A.shape[2])
-    Pr = U_ref.reshape(-1, U_ref.shape[2])
-    d_sub = np.linalg.norm(Pa @ (Pa.conj().T @ Pr) - Pr) / math.sqrt(Pr.shape[1])
-    report("config2 chosen", E=E_gpu, E_rel_diff=dE, D=new_A.shape[2], D_ref=Pr.shape[1],
-           basis_splits=lt.basis_splits, fallbacks=lt.fallbacks, subspace_diff=d_sub,
-           error=psi.error())
+    # the two eigenvectors agree up to a global phase and the solvers' tolerance
+    t_gpu, t_ref = AB.cpu().numpy().reshape(-1), AB_ref.reshape(-1)
+    ph = np.vdot(t_gpu, t_ref)
+    ph /= abs(ph)
+    d_theta = np.linalg.norm(t_gpu * ph - t_ref)
+    # the kept two-site tensors: each is within its truncation error of its
+    # theta, so they agree to the theta difference plus both errors (the kept
+    # subspace itself is not unique when sigma_D and sigma_{D+1} are close)
+    rec = (new_A.reshape(-1, D_new) @ new_B.reshape(D_new, -1)).reshape(-1) * ph
+    rec_ref = (U_ref.reshape(-1, U_ref.shape[2]) @ sV_ref.reshape(U_ref.shape[2], -1)).reshape(-1)
+    d_rec = np.linalg.norm(rec - rec_ref)
+    err = psi.error()  # sqrt of the discarded weight of this one split
+    report("config2 chosen", E=E_gpu, E_rel_diff=dE, D=D_new, D_ref=U_ref.shape[2],
+           basis_splits=lt.basis_splits, fallbacks=lt.fallbacks, theta_diff=d_theta,
+           kept_diff=d_rec, sqrt_err=err, sqrt_err_ref=err_ref)
     assert dE < 1e-9
-    assert new_A.shape[2] == Pr.shape[1] == chi
+    assert D_new == U_ref.shape[2]
     assert lt.calls == 1
-    assert d_sub < 1e-4  # vectors near the cut rotate with sigma_256 / sigma_257 gaps
+    assert abs(err - err_ref) <= d_theta + 1e-12
+    assert d_rec <= d_theta + err + err_ref + 1e-12
 
 
 def test_config2_xx_chain_free_fermions_n64_chi256(S):
@@ -227,7 +243,16 @@ def _qtt_problem(S, n, tol):
     Ly = D1.extend(2 * n, sites=[2 * k + 1 for k in range(n)])
     Lap = (Lx + Ly).join(strategy=st)
     perm = S.interleaving_permutation([n, n])
-    d1 = sp.csr_matrix(np.real_if_close(D1.to_matrix()))
+    # the operator of D1 as a sparse matrix: second differences with zero
+    # values outside the grid, checked against the MPO on random vectors
+    N1 = 2 ** n
+    d1 = sp.diags([np.ones(N1 - 1), -2.0 * np.ones(N1), np.ones(N1 - 1)], [-1, 0, 1],
+                  format="csr") / I.step ** 2
+    rng = np.random.default_rng(3)
+    for _ in range(2):
+        v = rng.normal(size=N1)
+        mv = D1.apply(S.MPS.from_vector(v, [2] * n, normalize=False), simplify=False)
+        assert rel(mv.to_vector(), d1 @ v) < 1e-10
     return psi, Lap, st, d1, np.outer(fx, fy), perm
 
 
@@ -258,7 +283,9 @@ def test_config3_qtt_laplacian_euler_n24(S):
             F = F + dt * (d1 @ F + (d1 @ F.T).T)
         return F
 
-    assert rel(_grid(psi0.to_vector(), n, perm), F0) < 1e-13
+    # loading truncates at the default tolerance eps on the squared norm:
+    # relative L2 error up to sqrt(eps) per factor
+    assert rel(_grid(psi0.to_vector(), n, perm), F0) < 1e-7
     # stated time step: 3 steps against the dense iteration and the oracle
     dt = 1e-6
     out = S.euler(Lap, (0.0, 3 * dt), psi0, steps=3, strategy=st)
@@ -268,13 +295,14 @@ def test_config3_qtt_laplacian_euler_n24(S):
     W = [A.cpu().numpy() for A in Lap]
     o = oa.OMPS([A.cpu().numpy() for A in psi0], 0.0)
     mine = psi0
-    d_or = []
+    d_or, bonds = [], []
     for _ in range(2):
         Lo = oa.mpo_apply(W, o, ost, simplify_flag=True)
         o = oa.simplify(oa.OMPSSum([1.0, dt], [o, Lo]), ost)
         mine = S.euler(Lap, (0.0, dt), mine, steps=1, strategy=st)
-        assert mine.bond_dimensions() == o.bond_dimensions()
+        bonds.append((mine.bond_dimensions(), o.bond_dimensions()))
         d_or.append(rel(mine.to_vector(), o.to_vector()))
+    report("config3 oracle", bonds=bonds, state_rel_diff=d_or)
     # stated workload to the end (reported: the exact iteration diverges too)
     out50 = S.euler(Lap, (0.0, 50 * dt), psi0, steps=50, strategy=st)
     F50 = exact(F0, dt, 50)
@@ -288,7 +316,15 @@ def test_config3_qtt_laplacian_euler_n24(S):
            exact_norm_growth=np.linalg.norm(F50) / np.linalg.norm(F0), err_stable_50=es,
            max_bond=outs.max_bond_dimension(), error_bound=outs.error() / np.linalg.norm(Fs))
     assert e3 < 1e-6
-    assert max(d_or) < 1e-9
+    # Both forms keep what the tolerance rule keeps; the function's mirror
+    # symmetry makes singular values come in equal pairs, so where such a pair
+    # sits at the threshold the two implementations may keep or drop it as a
+    # whole (bond dimensions differing by 2 at that bond).  Elsewhere the
+    # states agree to rounding; with a pair kept by one side only they agree
+    # to the truncation tolerance.
+    for (mb, ob_), d in zip(bonds, d_or):
+        assert all(abs(x - y) in (0, 2) for x, y in zip(mb, ob_))
+        assert d < (1e-9 if mb == ob_ else 1e-4)
     assert np.isfinite(n50)
     # simplify to 1e-9 (relative norm-squared) per step: the L2 error stays
     # within the accumulated truncation bound
