@@ -145,8 +145,8 @@ int svd_split_impl(int64_t m, int64_t n, const double2* A, int mode, int method,
 // Jacobi rotations applied as unitary polar factors (polar_svd.hip).
 int64_t polar_svd_bytes(int64_t Lg, int64_t l);
 int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, double* S,
-              double2* VH, const int64_t** tiny_dev, void* work, int64_t work_bytes,
-              int32_t* info, hipStream_t stream);
+              double2* VH, const int64_t** tiny_dev, const int** conv_dev, void* work,
+              int64_t work_bytes, hipStream_t stream);
 int qr_factor(double2* A, int64_t len, int64_t ncol, int nb_max, double2* Vall, double2* Tall,
               double2* tau, double2* Z, double2* ks, int64_t ks_elems, hipStream_t stream);
 int qr_apply_q(double2* Y, int64_t len, int64_t ny, int64_t ncol, int nb_max, const double2* Vall,

@@ -1,26 +1,33 @@
 // SVD of a matrix whose columns are already nearly orthogonal, by
-// simultaneous Jacobi rotations: every iteration forms the Gram matrix
-// G = W^H W with one MFMA GEMM, derives for every column pair (i, j) the
-// plane rotation that would annihilate G_ij (the one-sided Jacobi rotation
-// of that pair), assembles all of them into one matrix Z = I + X (X
-// anti-Hermitian, X_ij = t_ij ph_ij from the rotation's tangent and phase),
-// replaces Z by its unitary polar factor (Newton-Schulz iterations, GEMMs),
-// and rotates [W; V] <- [W; V] Z with one GEMM.  Near convergence the step
-// is Jacobi's: the normalised off-diagonal of G falls quadratically from one
-// iteration to the next, so a matrix that starts at 1e-3 needs four or five
-// iterations, each a handful of GEMMs on the matrix cores instead of a chain
-// of small dependent block steps.
+// simultaneous Jacobi rotations.  Every iteration forms the Gram matrix
+// G = W^H W, derives for every column pair (i, j) the plane rotation that
+// would annihilate G_ij (the one-sided Jacobi rotation of that pair: tangent
+// t and phase ph), assembles all of them into one matrix Z = I + X (X
+// anti-Hermitian, X_ij = t ph, X_ji = -t conj(ph)), replaces Z by its unitary
+// polar factor (Newton-Schulz iterations Z <- Z (3 I - Z^H Z) / 2 after a
+// scaling that puts its singular values in (0, 1]), and rotates
+// [W; V] <- [W; V] Z.  Near convergence the step is Jacobi's: the normalised
+// off-diagonal of G falls quadratically from one iteration to the next (a
+// captured chi = 256 DMRG split goes 2e-2, 4e-3, 5e-5, 1e-6, 5e-10), so a
+// handful of iterations of a few GEMMs each replace the long chain of small
+// dependent block steps of the block Jacobi.
 //
-// Accuracy: each G_ij is a dot product of two columns, with rounding
-// relative to the two column norms, so the rotations are as accurate as the
-// one-sided Jacobi's; the polar factor is unitary to a few ulp (the
-// Newton-Schulz count is chosen on the host from a bound on ||X||_2).
+// The whole iteration runs in one cooperative kernel: the products are 16 x 16
+// output tiles on FP64 MFMA spread over every workgroup, separated by grid
+// barriers, and the convergence measure is read on the device.
 //
-// Used by the basis-sketched split (sketch.hip), whose W = op(A) Omega has
-// nearly orthogonal columns; a matrix that does not converge within
-// POLAR_MAX_ITERS iterations is reported as not converged and the caller
-// falls back to the full SVD.
+// Accuracy: each G_ij is a dot product of two columns, rounded relative to the
+// two column norms, so the rotations are as accurate as one-sided Jacobi's;
+// the polar factor is unitary to a few ulp (the Newton-Schulz count follows
+// from a bound on ||X||_2).
+//
+// Used by the basis-sketched split at a capped bond (sketch.hip), whose
+// W = op(A) basis has nearly orthogonal columns; a matrix that does not
+// converge within POLAR_MAX_ITERS iterations is reported as such and the
+// caller uses the block Jacobi instead.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "common.h"
 #include "jacobi_inner.h"
@@ -48,8 +55,7 @@ struct PolarWork {
   double* nrm2;    // l squared column norms
   int* perm;       // l: columns by decreasing norm
   int64_t* tiny;   // first output slot with a negligible column (l if none)
-  double2* sk;     // split-K scratch
-  int64_t sk_elems;
+  void* ctrl;      // PolarCtrl of the cooperative iteration
   int64_t bytes;
 };
 
@@ -72,12 +78,7 @@ PolarWork polar_carve(int64_t Lg, int64_t l, void* base) {
   w.nrm2 = (double*)take(l * 8);
   w.perm = (int*)take(l * 4);
   w.tiny = (int64_t*)take(8);
-  int64_t sk = splitk_workspace(l, l, Lg);
-  const int64_t s2 = splitk_workspace(l, l, l), s3 = splitk_workspace(H, l, l);
-  if (s2 > sk) sk = s2;
-  if (s3 > sk) sk = s3;
-  w.sk_elems = sk;
-  w.sk = (double2*)take(sk * 16);
+  w.ctrl = take(256);
   w.bytes = off;
   return w;
 }
@@ -112,77 +113,6 @@ __global__ __launch_bounds__(1024) void polar_frob(int64_t Lg, int64_t l,
     stats[1] = 0.0;
     stats[2] = t;
     stats[3] = le * le * t;
-  }
-}
-
-__global__ void polar_reset(double* __restrict__ stats) {
-  if (threadIdx.x == 0) {
-    stats[0] = 0.0;
-    stats[1] = 0.0;
-  }
-}
-
-// Row i of Z = I + X from the Hermitian Gram matrix G (upper triangle used;
-// the lower one is its mirror): for i < j the pair (i, j) rotation of
-// one-sided Jacobi, J = [[cs, sn ph], [-sn conj(ph), cs]] on columns (i, j),
-// enters as X_ij = t ph and X_ji = -t conj(ph), t = sn / cs.  Pairs already
-// orthogonal to `tol` (relative to the two column norms), or with a
-// negligible column, get no rotation.  Also accumulates, by atomic maximum on
-// the bit patterns of non-negative doubles, the largest normalised
-// off-diagonal and the largest row sum of |X| (= ||X||_1 = ||X||_inf for an
-// anti-Hermitian X, a bound on ||X||_2).  One workgroup per row.
-__global__ __launch_bounds__(256) void polar_generator(int64_t l, const double2* __restrict__ G,
-                                                       double tol, double* __restrict__ stats,
-                                                       double2* __restrict__ Z) {
-  __shared__ double red_off[4], red_sum[4];
-  const int64_t i = blockIdx.x;
-  const double tiny2 = stats[3];
-  double mx = 0.0, rowsum = 0.0;
-  for (int64_t j = threadIdx.x; j < l; j += blockDim.x) {
-    double2 z = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    if (j != i) {
-      const int64_t a = i < j ? i : j, b = i < j ? j : i;
-      const double gaa = G[a * l + a].x, gbb = G[b * l + b].x;
-      const double2 c = G[a * l + b];
-      const double s2 = cabs2(c);
-      if (gaa > tiny2 && gbb > tiny2) mx = fmax(mx, sqrt(s2 / gaa / gbb));
-      if (s2 > 0.0 && gaa > tiny2 && gbb > tiny2 && s2 > (tol * tol * gaa) * gbb) {
-        const double r = jac::rsqrt_nr(s2);  // 1 / |G_ab|
-        const double2 ph = make_double2(c.x * r, c.y * r);
-        const double tau = (gbb - gaa) * (0.5 * r);
-        double t;
-        if (fabs(tau) > 1e150) {
-          t = 0.5 * jac::rcp_nr(tau);
-        } else {
-          const double q = fma(tau, tau, 1.0);
-          t = (tau >= 0.0 ? 1.0 : -1.0) * jac::rcp_nr(fabs(tau) + q * jac::rsqrt_nr(q));
-        }
-        // row i: X_ab = t ph (i = a), X_ba = -t conj(ph) (i = b)
-        z = (i == a) ? make_double2(t * ph.x, t * ph.y) : make_double2(-t * ph.x, t * ph.y);
-        rowsum += fabs(t);
-      }
-    }
-    Z[i * l + j] = z;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    mx = fmax(mx, __shfl_down(mx, off, 64));
-    rowsum += __shfl_down(rowsum, off, 64);
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    red_off[wave] = mx;
-    red_sum[wave] = rowsum;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = 0.0, s = 0.0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
-      m = fmax(m, red_off[q]);
-      s += red_sum[q];
-    }
-    // non-negative doubles order like their bit patterns
-    atomicMax((unsigned long long*)&stats[0], (unsigned long long)__double_as_longlong(m));
-    atomicMax((unsigned long long*)&stats[1], (unsigned long long)__double_as_longlong(s));
   }
 }
 
@@ -245,6 +175,312 @@ __global__ void polar_output(int64_t Lg, int64_t l, const double2* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// The whole iteration in one cooperative kernel: the GEMMs are spread over
+// all workgroups as 16 x 16 output tiles (one wavefront each, FP64 MFMA),
+// separated by grid barriers, and the convergence test is read on the device,
+// so no launch or host round trip sits between the small products.
+
+typedef double pd4 __attribute__((ext_vector_type(4)));
+
+struct PolarCtrl {
+  unsigned int barrier;   // grid barrier arrivals
+  unsigned int abort;     // a barrier wait timed out
+  unsigned int released;  // last barrier phase whose arrivals are complete
+  int iterations;
+  int converged;
+  unsigned long long stats[2][2];  // per iteration parity: max off, |X| bound (bits)
+  long long prof[4];  // workgroup 0: ticks waiting in barriers, ticks in all, barriers
+};
+
+constexpr unsigned long long POLAR_SPIN_LIMIT = 4000000ull;
+
+// Bounded grid barrier (all workgroups resident: cooperative launch).  The
+// last workgroup to arrive publishes the phase in a separate word, which the
+// others poll, so the waiters do not contend with the arrivals.  Returns false
+// once any workgroup has given up waiting.
+__device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int ph = ++*phase;
+    const unsigned int target = ph * gridDim.x;
+    const long long t0 = (blockIdx.x == 0) ? (long long)wall_clock64() : 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int before = atomicAdd(&c->barrier, 1u);
+    int ok = 1;
+    if (before + 1 == target) {
+      __hip_atomic_store(&c->released, ph, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      unsigned long long spins = 0;
+      while (__hip_atomic_load(&c->released, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ph) {
+        if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+        if (++spins > POLAR_SPIN_LIMIT) {
+          atomicExch(&c->abort, 1u);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (blockIdx.x == 0) {
+      c->prof[0] += (long long)wall_clock64() - t0;
+      c->prof[2] += 1;
+    }
+    *s_ok = ok;
+  } else {
+    ++*phase;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// Cout = alpha op(A) B + beta Cin (row-major; op(A) = A (M x K, stride lda) or
+// A^H with A stored K x M; B K x N; Cin may be null when beta = 0).  One
+// 16 x 16 output tile per workgroup at a time, tiles handed out over the
+// grid; the four wavefronts of the workgroup take a quarter of K each and
+// their partial tiles are summed in LDS in a fixed order.  Each wavefront
+// issues the loads of 16 K-steps before their MFMAs, so that one memory
+// latency covers 64 MFMAs.  MFMA f64 16x16x4: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator item r is D[l / 16 + 4 r][l % 16].
+constexpr int PG_UNROLL = 16;
+template <bool ADJ>
+__device__ void polar_gemm(int64_t M, int64_t N, int64_t K, const double2* __restrict__ A,
+                           int64_t lda, const double2* __restrict__ B, int64_t ldb, double alpha,
+                           double beta, const double2* __restrict__ Cin, double2* __restrict__ C,
+                           int64_t ldc, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t tm = ceil_div(M, 16), tn = ceil_div(N, 16);
+  const int li = lane & 15, lk = lane >> 4;
+  // K range of this wavefront, in whole K-steps of 4
+  const int64_t ksteps = ceil_div(K, 4);
+  const int64_t per = ceil_div(ksteps, 4);
+  const int64_t s0 = wave * per, s1 = (s0 + per < ksteps) ? s0 + per : ksteps;
+  for (int64_t t = blockIdx.x; t < tm * tn; t += gridDim.x) {
+    const int64_t m0 = (t / tn) * 16, n0 = (t % tn) * 16;
+    const int64_t ia = m0 + li, jb = n0 + li;
+    pd4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t sb = s0; sb < s1; sb += PG_UNROLL) {
+      double2 av[PG_UNROLL], bv[PG_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PG_UNROLL; ++u) {
+        const int64_t k = (sb + u) * 4 + lk;
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        if (sb + u < s1 && k < K) {
+          if (ADJ) {
+            if (ia < M) a = A[k * lda + ia];
+          } else if (ia < M) {
+            a = A[ia * lda + k];
+          }
+          if (jb < N) b = B[k * ldb + jb];
+        }
+        av[u] = a;
+        bv[u] = b;
+      }
+#pragma unroll
+      for (int u = 0; u < PG_UNROLL; ++u) {
+        const double ar = av[u].x, ai = ADJ ? -av[u].y : av[u].y;
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bv[u].x, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bv[u].y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bv[u].y, ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bv[u].x, ci, 0, 0, 0);
+      }
+    }
+    // partial tiles of waves 1..3 to LDS; wave 0 sums them in order and stores
+    if (wave > 0) {
+      double* dst = red + ((wave - 1) * 64 + lane) * 8;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        dst[r] = cr[r];
+        dst[4 + r] = ci[r];
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double* src = red + (q * 64 + lane) * 8;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          cr[r] += src[r];
+          ci[r] += src[4 + r];
+        }
+      }
+      const int64_t col = n0 + li;
+      if (col < N) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = m0 + lk + 4 * r;
+          if (row < M) {
+            double2 v = make_double2(alpha * cr[r], alpha * ci[r]);
+            if (beta != 0.0) {
+              const double2 c0 = Cin[row * ldc + col];
+              v.x += beta * c0.x;
+              v.y += beta * c0.y;
+            }
+            C[row * ldc + col] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ int ns_steps_device(double s) {
+  int k = 0;
+  while (1.0 - s > 2.3e-16 && k < 60) {
+    const double next = s * (3.0 - s * s) * 0.5;
+    ++k;
+    if (next == s) break;
+    s = next;
+  }
+  return k + 1;
+}
+
+// The rotation iteration of the file comment, all of it on the device.
+// XA holds [W; I] on entry and [W V; V] on exit (copied back from the
+// ping-pong buffer if needed); stats[3] is the negligible squared column norm.
+__global__ __launch_bounds__(256) void polar_persistent(int64_t Lg, int64_t l, double2* XA,
+                                                        double2* XB, double2* G, double2* Z,
+                                                        double2* Z2, double2* T,
+                                                        const double* __restrict__ stats,
+                                                        PolarCtrl* ctrl, double tol,
+                                                        int max_iters) {
+  __shared__ double red_off[4], red_sum[4];
+  __shared__ double red[3 * 64 * 8];
+  __shared__ int s_ok;
+  unsigned int phase = 0;
+  const long long t_start = (long long)wall_clock64();
+  const int64_t H = Lg + l;
+  const double tiny2 = stats[3];
+  double2* cur = XA;
+  double2* nxt = XB;
+  int it = 0, converged = 0;
+  for (it = 0; it < max_iters; ++it) {
+    // G = W^H W
+    polar_gemm<true>(l, l, Lg, cur, l, cur, l, 1.0, 0.0, nullptr, G, l, red);
+    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
+    // Z = I + X, rows over the workgroups; the statistics of this parity
+    // were cleared one iteration ago
+    unsigned long long* st = ctrl->stats[it & 1];
+    for (int64_t i = blockIdx.x; i < l; i += gridDim.x) {
+      double mx = 0.0, rowsum = 0.0;
+      for (int64_t j = threadIdx.x; j < l; j += blockDim.x) {
+        double2 z = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        if (j != i) {
+          const int64_t a = i < j ? i : j, b = i < j ? j : i;
+          const double gaa = G[a * l + a].x, gbb = G[b * l + b].x;
+          const double2 c = G[a * l + b];
+          const double s2 = cabs2(c);
+          if (gaa > tiny2 && gbb > tiny2) mx = fmax(mx, sqrt(s2 / gaa / gbb));
+          if (s2 > 0.0 && gaa > tiny2 && gbb > tiny2 && s2 > (tol * tol * gaa) * gbb) {
+            const double r = jac::rsqrt_nr(s2);
+            const double2 ph = make_double2(c.x * r, c.y * r);
+            const double tau = (gbb - gaa) * (0.5 * r);
+            double t;
+            if (fabs(tau) > 1e150) {
+              t = 0.5 * jac::rcp_nr(tau);
+            } else {
+              const double q = fma(tau, tau, 1.0);
+              t = (tau >= 0.0 ? 1.0 : -1.0) * jac::rcp_nr(fabs(tau) + q * jac::rsqrt_nr(q));
+            }
+            z = (i == a) ? make_double2(t * ph.x, t * ph.y) : make_double2(-t * ph.x, t * ph.y);
+            rowsum += fabs(t);
+          }
+        }
+        Z[i * l + j] = z;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_down(mx, off, 64));
+        rowsum += __shfl_down(rowsum, off, 64);
+      }
+      if ((threadIdx.x & 63) == 0) {
+        red_off[threadIdx.x >> 6] = mx;
+        red_sum[threadIdx.x >> 6] = rowsum;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double m = 0.0, su = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+          m = fmax(m, red_off[q]);
+          su += red_sum[q];
+        }
+        atomicMax(&st[0], (unsigned long long)__double_as_longlong(m));
+        atomicMax(&st[1], (unsigned long long)__double_as_longlong(su));
+      }
+      __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicExch(&ctrl->stats[(it + 1) & 1][0], 0ull);
+      atomicExch(&ctrl->stats[(it + 1) & 1][1], 0ull);
+    }
+    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
+    const double off = __longlong_as_double(
+        (long long)__hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const double xn = __longlong_as_double(
+        (long long)__hip_atomic_load(&st[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (!(off == off)) break;  // NaN: not converged
+    if (off < tol || xn == 0.0) {
+      converged = 1;
+      break;
+    }
+    // polar factor of f Z, f scaling its singular values into (0, 1]; the
+    // first Newton-Schulz step applies f through its coefficients:
+    // T = f^2 Z^H Z, Z2 = 1.5 f Z - 0.5 f Z T
+    const double f = 1.0 / sqrt(1.0 + xn * xn);
+    const int ns = ns_steps_device(f);
+    bool ok = true;
+    for (int k = 0; k < ns && ok; ++k) {
+      const double g = (k == 0) ? f : 1.0;
+      polar_gemm<true>(l, l, l, Z, l, Z, l, g * g, 0.0, nullptr, T, l, red);
+      if (!(ok = polar_barrier(ctrl, &phase, &s_ok))) break;
+      polar_gemm<false>(l, l, l, Z, l, T, l, -0.5 * g, 1.5 * g, Z, Z2, l, red);
+      if (!(ok = polar_barrier(ctrl, &phase, &s_ok))) break;
+      double2* tmp = Z;
+      Z = Z2;
+      Z2 = tmp;
+    }
+    if (!ok) break;
+    polar_gemm<false>(H, l, l, cur, l, Z, l, 1.0, 0.0, nullptr, nxt, l, red);  // [W; V] Z
+    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
+    double2* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    if (off < POLAR_QUAD_STOP) {
+      converged = 1;
+      ++it;
+      break;
+    }
+  }
+  if (cur != XA) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < H * l;
+         e += (int64_t)gridDim.x * blockDim.x)
+      XA[e] = cur[e];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctrl->prof[1] = (long long)wall_clock64() - t_start;
+    ctrl->iterations = it;
+    ctrl->converged = converged && !__hip_atomic_load(&ctrl->abort, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void polar_ctrl_reset(PolarCtrl* c) {
+  if (threadIdx.x == 0) {
+    c->barrier = 0;
+    c->abort = 0;
+    c->released = 0;
+    c->iterations = 0;
+    c->converged = 0;
+    c->stats[0][0] = c->stats[0][1] = c->stats[1][0] = c->stats[1][1] = 0ull;
+    c->prof[0] = c->prof[1] = c->prof[2] = c->prof[3] = 0;
+  }
+}
+
 unsigned polar_grid(int64_t n) {
   int64_t b = ceil_div(n, 256);
   if (b > 2048) b = 2048;
@@ -252,109 +488,69 @@ unsigned polar_grid(int64_t n) {
   return (unsigned)b;
 }
 
-// Newton-Schulz steps that take the smallest singular value s of the scaled
-// Z (all singular values lie in [s, 1]) to 1 within rounding, plus one.
-int newton_schulz_steps(double s) {
-  int k = 0;
-  while (1.0 - s > 1e-16 && k < 60) {
-    s = s * (3.0 - s * s) * 0.5;
-    ++k;
-  }
-  return k + 1;
-}
-
 }  // namespace
 
 int64_t polar_svd_bytes(int64_t Lg, int64_t l) { return polar_carve(Lg, l, nullptr).bytes; }
 
 // SVD W = U diag(S) VH of the Lg x l matrix W (row stride ldw, Lg >= l, columns
-// nearly orthogonal): U Lg x l (row stride l), S descending, VH l x l.  The
-// stream is synchronised once per iteration (the host reads the convergence
-// measure).  *tiny_dev (device, may be read after the caller's next
-// synchronisation) receives the first slot whose singular value is
-// negligible (<= Lg eps ||W||_F), l if none: such slots have U columns of
-// zero, which the caller must not keep.  Returns SEEMPS_ERR_NOCONVERGE when
-// POLAR_MAX_ITERS iterations do not reach the tolerance.  info = {iterations,
-// converged}.
+// nearly orthogonal): U Lg x l (row stride l), S descending, VH l x l.  All
+// work is enqueued on the stream, nothing is synchronised.  Device results the
+// caller reads at its next synchronisation: *conv_dev (int) = 1 when the
+// iteration converged (otherwise U, S, VH must not be used), *tiny_dev
+// (int64) = the first slot whose singular value is negligible
+// (<= Lg eps ||W||_F), l if none: such slots have U columns of zero, which the
+// caller must not keep.  Returns SEEMPS_ERR_NOCONVERGE without enqueuing
+// anything when the cooperative grid cannot be resident.
 int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, double* S,
-              double2* VH, const int64_t** tiny_dev, void* work, int64_t work_bytes,
-              int32_t* info, hipStream_t st_) {
+              double2* VH, const int64_t** tiny_dev, const int** conv_dev, void* work,
+              int64_t work_bytes, hipStream_t st_) {
   SEEMPS_CHECK_ARG(Lg >= l && l >= 1, "polar_svd: needs Lg >= l >= 1");
   PolarWork w = polar_carve(Lg, l, work);
   SEEMPS_CHECK_ARG(work_bytes >= w.bytes, "polar_svd: workspace too small");
-  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  static_assert(sizeof(PolarCtrl) <= 256, "control block too large");
   const int64_t H = Lg + l;
   const double tol = 32.0 * sqrt((double)Lg) * 2.220446049250313e-16;
-  double* hp = (double*)pinned_stage(PINNED_BYTES);
-  SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
-  int st;
+  DeviceInfo di;
+  int st = device_info(&di);
+  if (st) return st;
+  static thread_local int occ_dev = -1, occ = 0;
+  if (occ_dev != di.device) {
+    int o = 0;
+    SEEMPS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)polar_persistent,
+                                                            256, 0));
+    occ = o;
+    occ_dev = di.device;
+  }
+  // one workgroup per CU (the widest product has ceil(H/16) ceil(l/16) tiles
+  // of one workgroup each)
+  int64_t grid = ceil_div(H, 16) * ceil_div(l, 16);
+  if (grid > di.num_cu) grid = di.num_cu;
+  if (grid > (int64_t)occ * di.num_cu) grid = (int64_t)occ * di.num_cu;
+  if (grid < 1) {
+    set_error("polar_svd: cooperative kernel cannot be resident");
+    return SEEMPS_ERR_NOCONVERGE;
+  }
   hipLaunchKernelGGL(polar_init, dim3(polar_grid(H * l)), dim3(256), 0, st_, Lg, l, W, ldw,
                      w.XA);
   if ((st = check_launch("polar_init"))) return st;
   hipLaunchKernelGGL(polar_frob, dim3(1), dim3(1024), 0, st_, Lg, l, w.XA, w.stats);
   if ((st = check_launch("polar_frob"))) return st;
-  int it = 0;
-  bool converged = false;
-  for (it = 0; it < POLAR_MAX_ITERS; ++it) {
-    if (it > 0) {
-      hipLaunchKernelGGL(polar_reset, dim3(1), dim3(64), 0, st_, w.stats);
-      if ((st = check_launch("polar_reset"))) return st;
-    }
-    // G = W^H W
-    if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_N, l, l, Lg, one, w.XA, l, w.XA, l, zero, w.G, l,
-                       w.sk, w.sk_elems, st_)))
-      return st;
-    {
-      TimedLaunch timed("polar_generator", st_, 0.0);
-      hipLaunchKernelGGL(polar_generator, dim3((unsigned)l), dim3(256), 0, st_, l, w.G, tol,
-                         w.stats, w.Z);
-      if ((st = check_launch("polar_generator"))) return st;
-    }
-    SEEMPS_HIP(hipMemcpyAsync(hp, w.stats, 16, hipMemcpyDeviceToHost, st_));
-    SEEMPS_HIP(hipStreamSynchronize(st_));
-    const double off = hp[0], xnorm = hp[1];
-    if (!(off == off)) break;  // NaN: not converged
-    if (off < tol || xnorm == 0.0) {
-      converged = true;
-      break;
-    }
-    // polar factor of Z: scaled so its singular values lie in (0, 1]
-    const double f = 1.0 / sqrt(1.0 + xnorm * xnorm);
-    if ((st = zaxpby_impl(l * l, make_double2(f, 0.0), w.Z, 0, zero, w.Z, st_))) return st;
-    const int ns = newton_schulz_steps(f);
-    for (int k = 0; k < ns; ++k) {
-      // T = Z^H Z; Z2 = 1.5 Z - 0.5 Z T
-      if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_N, l, l, l, one, w.Z, l, w.Z, l, zero, w.T, l,
-                         w.sk, w.sk_elems, st_)))
-        return st;
-      if ((st = zaxpby_impl(l * l, make_double2(1.5, 0.0), w.Z, 0, zero, w.Z2, st_))) return st;
-      if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, l, l, l, make_double2(-0.5, 0.0), w.Z, l,
-                         w.T, l, one, w.Z2, l, w.sk, w.sk_elems, st_)))
-        return st;
-      double2* t = w.Z;
-      w.Z = w.Z2;
-      w.Z2 = t;
-    }
-    // [W; V] <- [W; V] Z
-    if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, H, l, l, one, w.XA, l, w.Z, l, zero, w.XB, l,
-                       w.sk, w.sk_elems, st_)))
-      return st;
-    double2* t = w.XA;
-    w.XA = w.XB;
-    w.XB = t;
-    if (off < POLAR_QUAD_STOP) {
-      converged = true;
-      ++it;
-      break;
-    }
-  }
-  if (info) {
-    info[0] = it;
-    info[1] = converged ? 1 : 0;
-  }
-  if (!converged) {
-    set_error("polar_svd: not converged after %d iterations", it);
-    return SEEMPS_ERR_NOCONVERGE;
+  PolarCtrl* ctrl = (PolarCtrl*)w.ctrl;
+  hipLaunchKernelGGL(polar_ctrl_reset, dim3(1), dim3(64), 0, st_, ctrl);
+  if ((st = check_launch("polar_ctrl_reset"))) return st;
+  {
+    int64_t Lg_ = Lg, l_ = l;
+    double tol_ = tol;
+    int iters = POLAR_MAX_ITERS;
+    const double* stats = w.stats;
+    void* args[] = {(void*)&Lg_, (void*)&l_, (void*)&w.XA, (void*)&w.XB, (void*)&w.G,
+                    (void*)&w.Z, (void*)&w.Z2, (void*)&w.T, (void*)&stats, (void*)&ctrl,
+                    (void*)&tol_, (void*)&iters};
+    // nominal SVD flops of the matrix (as for svd_jacobi): 4 (6 Lg l^2 + 16 l^3)
+    const double M_ = (double)Lg, N_ = (double)l;
+    TimedLaunch timed("polar_svd", st_, 4.0 * (6.0 * M_ * N_ * N_ + 16.0 * N_ * N_ * N_));
+    SEEMPS_HIP(hipLaunchCooperativeKernel((const void*)polar_persistent, dim3((unsigned)grid),
+                                          dim3(256), args, 0, st_));
   }
   hipLaunchKernelGGL(polar_norms, dim3((unsigned)l), dim3(256), 0, st_, Lg, l, w.XA, w.nrm2);
   if ((st = check_launch("polar_norms"))) return st;
@@ -366,7 +562,21 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
   hipLaunchKernelGGL(polar_output, dim3(polar_grid(H * l)), dim3(256), 0, st_, Lg, l, w.XA,
                      w.nrm2, w.perm, U, S, VH);
   if ((st = check_launch("polar_output"))) return st;
+  static const bool trace = [] {
+    const char* e = getenv("SEEMPS_POLAR_TRACE");
+    return e && e[0] == '1';
+  }();
+  if (trace) {  // diagnostics: synchronises
+    PolarCtrl h;
+    SEEMPS_HIP(hipMemcpyAsync(&h, ctrl, sizeof(h), hipMemcpyDeviceToHost, st_));
+    SEEMPS_HIP(hipStreamSynchronize(st_));
+    fprintf(stderr, "[polar_svd %lld x %lld] %d iterations, converged %d, %lld barriers, "
+            "%.1f us of %.1f us waiting in barriers (workgroup 0), grid %lld\n", (long long)Lg,
+            (long long)l, h.iterations, h.converged, h.prof[2], h.prof[0] * 0.01, h.prof[1] * 0.01,
+            (long long)grid);
+  }
   if (tiny_dev) *tiny_dev = w.tiny;
+  if (conv_dev) *conv_dev = &ctrl->converged;
   return SEEMPS_OK;
 }
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <initializer_list>
 
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -753,64 +754,99 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
     }
   }
   int off = 0;
-  // W has nearly orthogonal columns: by default its SVD is the simultaneous
-  // rotation iteration of polar_svd.hip (a few GEMM iterations); the block
-  // Jacobi (SEEMPS_BASIS_SVD=jacobi) is the alternative.  A polar iteration
-  // that does not converge hands the split to the full SVD.
-  static const bool use_polar = [] {
+  // W = X [basis | expansion].  When the sketch is the basis alone (p = 0: a
+  // bond at its cap), W is the new two-site matrix seen through the previous
+  // singular vectors and its columns are orthogonal up to the change of the
+  // two-site tensor: its SVD is the simultaneous rotation iteration of
+  // polar_svd.hip, a few GEMM iterations.  Expansion columns are not
+  // orthogonal to the rest, and the block Jacobi (which converges from any
+  // start) takes those matrices, and any the rotation iteration does not
+  // finish.  SEEMPS_BASIS_SVD=jacobi uses the block Jacobi throughout.
+  static const bool polar_enabled = [] {
     const char* e = getenv("SEEMPS_BASIS_SVD");
     return !(e && strcmp(e, "jacobi") == 0);
   }();
-  const int64_t* tiny_dev = nullptr;
-  {
-    TimedLaunch phase("phase_basis_svd", st_, 0.0);
-    if (use_polar) {
-      st = polar_svd(Lg, l, w.W, l, w.Uw, w.S, w.VHw, &tiny_dev, w.svdw, w.svd_bytes, info, st_);
-      if (st == SEEMPS_ERR_NOCONVERGE) return full_split();
-      if (st) return st;
-    } else if ((st = svd_enqueue(Lg, l, w.W, l, w.Uw, w.S, w.VHw, w.svdw, w.svd_bytes, st_, &off,
-                                 false, false))) {
-      return st;
-    }
-  }
-  if ((st = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return st;
-  // R = A - (projection of A on Omega): side 0: A - W Oh; side 1: A - Oh^H W^H
-  SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
-  if (side == 0) {
-    if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, m, n, l, mone, w.W, l, w.Oh, n, one, w.Rres, n,
-                       w.sk, w.sk_elems, st_)))
-      return st;
-  } else {
-    if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_H, m, n, l, mone, w.Oh, m, w.W, l, one, w.Rres, n,
-                       w.sk, w.sk_elems, st_)))
-      return st;
-  }
-  if ((st = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return st;
-  hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
-                     w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
-  if ((st = check_launch("truncate_sketch"))) return st;
   char* hp = (char*)pinned_stage(PINNED_BYTES);
   SEEMPS_CHECK_ARG(hp != nullptr, "pinned host staging could not be allocated");
-  int64_t htiny = l;
-  if (use_polar) {
-    SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
-    SEEMPS_HIP(hipMemcpyAsync(hp + 24, tiny_dev, 8, hipMemcpyDeviceToHost, st_));
-    SEEMPS_HIP(hipStreamSynchronize(st_));
-    memcpy(&htiny, hp + 24, 8);
-  } else {
-    if ((st = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return st;
-    SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
-    if ((st = svd_read_ctrl(w.svdw, off, info, st_))) return st;  // synchronises
-  }
-  int64_t hD = 0;
+  int64_t hD = 0, htiny = l;
   double herr = 0.0;
-  int hfb = 0;
-  memcpy(&hD, hp, 8);
-  memcpy(&herr, hp + 8, 8);
-  memcpy(&hfb, hp + 16, sizeof(int));
+  int hfb = 0, hconv = 1;
+  // SVD of W, residual of the projection, truncation on the sketch spectrum,
+  // then one synchronisation that reads D, the discarded weight, the
+  // fallback flag and (rotation iteration) its convergence and first
+  // negligible slot.
+  auto svd_and_truncate = [&](bool polar) -> int {
+    int e;
+    const int64_t* tiny_dev = nullptr;
+    const int* conv_dev = nullptr;
+    {
+      TimedLaunch phase("phase_basis_svd", st_, 0.0);
+      if (polar) {
+        if ((e = polar_svd(Lg, l, w.W, l, w.Uw, w.S, w.VHw, &tiny_dev, &conv_dev, w.svdw,
+                           w.svd_bytes, st_)))
+          return e;
+      } else if ((e = svd_enqueue(Lg, l, w.W, l, w.Uw, w.S, w.VHw, w.svdw, w.svd_bytes, st_,
+                                  &off, false, false))) {
+        return e;
+      }
+    }
+    if ((e = zmdotc_impl(m * n, 1, A, m * n, A, w.frob2, w.red, st_))) return e;
+    // R = A - (projection of A on Omega): side 0: A - W Oh; side 1: A - Oh^H W^H
+    SEEMPS_HIP(hipMemcpyAsync(w.Rres, A, m * n * 16, hipMemcpyDeviceToDevice, st_));
+    if (side == 0) {
+      if ((e = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_N, m, n, l, mone, w.W, l, w.Oh, n, one, w.Rres, n,
+                        w.sk, w.sk_elems, st_)))
+        return e;
+    } else {
+      if ((e = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_H, m, n, l, mone, w.Oh, m, w.W, l, one, w.Rres, n,
+                        w.sk, w.sk_elems, st_)))
+        return e;
+    }
+    if ((e = zmdotc_impl(m * n, 1, w.Rres, m * n, w.Rres, w.resid2, w.red, st_))) return e;
+    hipLaunchKernelGGL(truncate_sketch_kernel, dim3(1), dim3(64), 0, st_, l, r, w.S, w.frob2,
+                       w.resid2, method, tolerance, max_bond, w.Ddev, w.edev, w.fdev);
+    if ((e = check_launch("truncate_sketch"))) return e;
+    if (polar) {
+      SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+      SEEMPS_HIP(hipMemcpyAsync(hp + 24, tiny_dev, 8, hipMemcpyDeviceToHost, st_));
+      SEEMPS_HIP(hipMemcpyAsync(hp + 32, conv_dev, sizeof(int), hipMemcpyDeviceToHost, st_));
+      SEEMPS_HIP(hipStreamSynchronize(st_));
+      memcpy(&htiny, hp + 24, 8);
+      memcpy(&hconv, hp + 32, sizeof(int));
+      if (info) {
+        info[0] = 0;
+        info[1] = hconv;
+      }
+    } else {
+      if ((e = svd_complete_enqueue(Lg, l, w.Uw, w.VHw, w.svdw, w.Ddev, st_, false))) return e;
+      SEEMPS_HIP(hipMemcpyAsync(hp, w.Ddev, 20, hipMemcpyDeviceToHost, st_));
+      if ((e = svd_read_ctrl(w.svdw, off, info, st_))) return e;  // synchronises
+      htiny = l;
+      hconv = 1;
+    }
+    memcpy(&hD, hp, 8);
+    memcpy(&herr, hp + 8, 8);
+    memcpy(&hfb, hp + 16, sizeof(int));
+    return SEEMPS_OK;
+  };
+  bool use_polar = polar_enabled && p == 0;
+  if ((st = svd_and_truncate(use_polar))) return st;
+  if (use_polar && !hconv) {
+    if (getenv("SEEMPS_POLAR_TRACE"))
+      fprintf(stderr, "[basis split %lld x %lld, k %lld] rotation iteration not converged: "
+              "block Jacobi\n", (long long)m, (long long)n, (long long)k);
+    use_polar = false;
+    if ((st = svd_and_truncate(false))) return st;
+  }
   // a kept slot with a negligible singular value has no vector from the
   // rotation iteration: the full SVD (with its orthonormal completion) decides
-  if (hfb || htiny < hD) return full_split();
+  if (hfb || htiny < hD) {
+    if (use_polar && getenv("SEEMPS_POLAR_TRACE"))
+      fprintf(stderr, "[basis split %lld x %lld, k %lld, p %lld] fallback: %s (D %lld, tiny %lld)\n",
+              (long long)m, (long long)n, (long long)k, (long long)p,
+              hfb ? "sketch weight" : "negligible kept value", (long long)hD, (long long)htiny);
+    return full_split();
+  }
   const int64_t D = hD;
   const double* sA = (mode == 1) ? w.S : nullptr;  // S into the left factor
   const double* sB = (mode == 0) ? w.S : nullptr;  // S into the right factor
