@@ -244,7 +244,7 @@ __device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok) {
 // grid; the four wavefronts of the workgroup take a quarter of K each and
 // their partial tiles are summed in LDS in a fixed order.  Each wavefront
 // issues the loads of 16 K-steps before their MFMAs, so that one memory
-// latency covers 64 MFMAs.  MFMA f64 16x16x4: lane l supplies A[l % 16][l / 16]
+// latency covers 64 MFMAs (a deeper batch of 24 measured slower).  MFMA f64 16x16x4: lane l supplies A[l % 16][l / 16]
 // and B[l / 16][l % 16]; accumulator item r is D[l / 16 + 4 r][l % 16].
 constexpr int PG_UNROLL = 16;
 template <bool ADJ>
@@ -524,6 +524,8 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
   // one workgroup per CU (the widest product has ceil(H/16) ceil(l/16) tiles
   // of one workgroup each)
   int64_t grid = ceil_div(H, 16) * ceil_div(l, 16);
+  // One workgroup per CU: two per CU (where resident) measured slower, the
+  // grid barriers costing more than the extra tiles save.
   if (grid > di.num_cu) grid = di.num_cu;
   if (grid > (int64_t)occ * di.num_cu) grid = (int64_t)occ * di.num_cu;
   if (grid < 1) {
@@ -571,9 +573,9 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
     SEEMPS_HIP(hipMemcpyAsync(&h, ctrl, sizeof(h), hipMemcpyDeviceToHost, st_));
     SEEMPS_HIP(hipStreamSynchronize(st_));
     fprintf(stderr, "[polar_svd %lld x %lld] %d iterations, converged %d, %lld barriers, "
-            "%.1f us of %.1f us waiting in barriers (workgroup 0), grid %lld\n", (long long)Lg,
-            (long long)l, h.iterations, h.converged, h.prof[2], h.prof[0] * 0.01, h.prof[1] * 0.01,
-            (long long)grid);
+            "%.1f us of %.1f us waiting in barriers (workgroup 0), grid %lld (%d per CU "
+            "resident)\n", (long long)Lg, (long long)l, h.iterations, h.converged, h.prof[2],
+            h.prof[0] * 0.01, h.prof[1] * 0.01, (long long)grid, occ);
   }
   if (tiny_dev) *tiny_dev = w.tiny;
   if (conv_dev) *conv_dev = &ctrl->converged;

@@ -272,11 +272,56 @@ __global__ void adjoint_scale(int64_t rows, int64_t cols, const double2* __restr
   }
 }
 
+// Eigenvectors of the p x p Hermitian Gram matrix G2 (p <= PW, row-major) of
+// the expansion block, by cyclic Jacobi sweeps in LDS (one workgroup of 256
+// threads; padding rows and columns are zero and get no rotation), into Q2
+// (p x p, row-major, unitary): the expansion directions rotated by Q2 give the
+// sketched matrix columns that are orthogonal among themselves.
+__global__ __launch_bounds__(256) void expansion_rotation(int p, const double2* __restrict__ G2,
+                                                          double2* __restrict__ Q2) {
+  using jac::GP;
+  using jac::PW;
+  __shared__ double2 g[PW * GP], g2[PW * GP], q[PW * GP], rot[PW];
+  __shared__ int active[2];
+  __shared__ double dmax;
+  for (int e = threadIdx.x; e < PW * PW; e += blockDim.x) {
+    const int i = e / PW, j = e % PW;
+    g[i * GP + j] = (i < p && j < p) ? G2[i * p + j] : make_double2(0.0, 0.0);
+    q[i * GP + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int i = 0; i < p; ++i) m = fmax(m, g[i * GP + i].x);
+    dmax = m;
+  }
+  __syncthreads();
+  // columns below (PW eps)^2 of the largest are left alone; pairs orthogonal
+  // to 1e-15 relative get no rotation
+  const double tiny2 = (double)PW * PW * 4.9e-32 * dmax;
+  double2* cur = g;
+  double2* other = g2;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    int nact = 0;
+    double2* res = jac::inner_jacobi(cur, other, q, rot, active, 1e-15, tiny2, 1, &nact);
+    if (res != cur) {
+      other = cur;
+      cur = res;
+    }
+    if (nact == 0) break;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < p * p; e += blockDim.x) {
+    const int i = e / p, j = e % p;
+    Q2[e] = q[i * GP + j];
+  }
+}
+
 // Workspace of the basis-sketched split: Lg x Sh working orientation (Lg = m,
 // Sh = n for side 0; Lg = n, Sh = m for side 1), l = k + p sketch columns.
 struct BasisWork {
   double2 *G0t, *Yt, *Zt, *C, *Qz, *Vall, *Tall, *tau, *Z, *ks, *Oh, *W, *Uw, *VHw, *T, *red,
-      *frob2, *Rres, *resid2;
+      *frob2, *Rres, *resid2, *G2, *Q2;
   double* S;
   int64_t* Ddev;
   double* edev;
@@ -324,11 +369,15 @@ BasisWork basis_carve(int64_t Lg, int64_t Sh, int64_t k, int64_t p, void* base) 
     w.edev = rk ? (double*)(rk + 8) : nullptr;
     w.fdev = rk ? (int*)(rk + 16) : nullptr;
   }
+  w.G2 = (double2*)take(pp * pp * 16);
+  w.Q2 = (double2*)take(pp * pp * 16);
   w.svd_bytes = svd_workspace_bytes(Lg, l, false);
   if (polar_svd_bytes(Lg, l) > w.svd_bytes) w.svd_bytes = polar_svd_bytes(Lg, l);
   w.svdw = take(w.svd_bytes);
   // the factor products (D <= l columns) split when D = l fits
   w.sk_elems = max_of({splitk_workspace(pp, Lg, Sh), splitk_workspace(pp, Sh, Lg),
+                       splitk_workspace(Lg, pp, Sh), splitk_workspace(pp, pp, Lg),
+                       splitk_workspace(pp, Sh, pp),
                        splitk_workspace(pp, k, Sh), splitk_workspace(pp, Sh, k),
                        splitk_workspace(Lg, l, Sh), splitk_workspace(Lg, Sh, l),
                        splitk_workspace(l, Sh, l), splitk_workspace(Sh, l, l)});
@@ -738,6 +787,31 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
       if ((st = project(w.Qz)) || (st = project(w.Qz))) return st;
       if ((st = orthonormalize(w.Qz, Sh, p, w.Zt, w, st_))) return st;
     }
+    if (p > 0 && p <= jac::PW) {
+      // Rotate the expansion directions so that their images X Z are
+      // orthogonal among themselves (they are already nearly orthogonal to
+      // the images of the basis): W = X [basis | Z] then has nearly orthogonal
+      // columns throughout and its SVD is the rotation iteration's.
+      // W2 = X Z (Lg x p) in Yt, G2 = W2^H W2, Q2 its eigenvectors, Zt <- Q2^T Zt.
+      if (side == 0) {
+        if ((st = zgemm_ws(SEEMPS_OP_N, SEEMPS_OP_T, m, p, n, one, A, n, w.Zt, n, zero, w.Yt, p,
+                           w.sk, w.sk_elems, st_)))
+          return st;
+      } else {
+        if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_T, n, p, m, one, A, n, w.Zt, m, zero, w.Yt, p,
+                           w.sk, w.sk_elems, st_)))
+          return st;
+      }
+      if ((st = zgemm_ws(SEEMPS_OP_H, SEEMPS_OP_N, p, p, Lg, one, w.Yt, p, w.Yt, p, zero, w.G2, p,
+                         w.sk, w.sk_elems, st_)))
+        return st;
+      hipLaunchKernelGGL(expansion_rotation, dim3(1), dim3(256), 0, st_, (int)p, w.G2, w.Q2);
+      if ((st = check_launch("expansion_rotation"))) return st;
+      if ((st = zgemm_ws(SEEMPS_OP_T, SEEMPS_OP_N, p, Sh, p, one, w.Q2, p, w.Zt, Sh, zero, w.G0t,
+                         Sh, w.sk, w.sk_elems, st_)))
+        return st;
+      SEEMPS_HIP(hipMemcpyAsync(w.Zt, w.G0t, p * Sh * 16, hipMemcpyDeviceToDevice, st_));
+    }
     const double2* Qext = w.Zt;  // p orthonormal columns orthogonal to the basis
     hipLaunchKernelGGL(build_omega_h, dim3(grid_for(l * Sh)), dim3(256), 0, st_, k, p, Sh, side,
                        Bs, Qext, w.Oh);
@@ -829,7 +903,7 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
     memcpy(&hfb, hp + 16, sizeof(int));
     return SEEMPS_OK;
   };
-  bool use_polar = polar_enabled && p == 0;
+  bool use_polar = polar_enabled && p <= jac::PW;
   if ((st = svd_and_truncate(use_polar))) return st;
   if (use_polar && !hconv) {
     if (getenv("SEEMPS_POLAR_TRACE"))

@@ -38,6 +38,8 @@ from ._lib import check, lib, ptr, stream
 
 F_IN = 32
 HIDDEN = 64
+#: Largest number of expansion directions of a basis-sketched split.
+MAX_EXPANSION = 32
 
 
 def spectral_features(AA: torch.Tensor, m: int, n: int, seed: int = 0) -> torch.Tensor:
@@ -199,7 +201,12 @@ class LearnedTruncation:
         feats = spectral_features(AA2, m, n, seed=self.seed + self.calls)
         l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
         if k is not None:
-            p = max(self.oversample, l - k)
+            # at most MAX_EXPANSION new directions per split: the expansion
+            # block is then rotated to orthogonal images in one workgroup and
+            # the sketched SVD stays the fast rotation iteration; a bond that
+            # needs to grow further is caught by the truncation (the weight
+            # outside the sketch exceeds the budget) and split in full
+            p = min(max(self.oversample, l - k), MAX_EXPANSION)
             if k + p < r:
                 return self._basis_split(AA2, m, n, mode, basis, k, p, strategy)
         if l >= r:
