@@ -60,6 +60,10 @@ def parse():
                     help="reduce the two --pmc passes of --pmc-workload (no GPU work)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-overlap measurement of many uneven states")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the fixed-width ablation of the learned predictor")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the timed drop-in dmrg() calls")
     ap.add_argument("--no-simplify", action="store_true",
                     help="skip the simplify-at-chi comparison of the two forms")
     ap.add_argument("--eig-precision", default="mixed", choices=["fp64", "mixed"],
@@ -107,6 +111,7 @@ def rocprof_summary(directory, bench_json=None, top=25):
         line = json.loads(open(bench_json).read().strip().splitlines()[-1])
         roof = line.get("roofline") or {}
         kern, nlaunch = roof.get("kernel"), roof.get("launches")
+        kern = ROCPROF_NAMES.get(kern, kern)
         if kern and nlaunch:
             durs = []
             for r in csv.DictReader(open(trace[0])):
@@ -124,44 +129,49 @@ def rocprof_summary(directory, bench_json=None, top=25):
 
 PMC_COPY_ELEMS = 4096 * 4096   # complex128 elements of the calibration copy (256 MiB)
 PMC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                           "svd_jacobi_dmrg_chi256_pmc_traffic.md")
+                           "dmrg_chi256_pmc_traffic.md")
+PMC_KERNELS = ("zgemm_kernel", "polar_svd", "polar_persistent", "svd_jacobi", "zlocal_kernel",
+               "lz_orth1", "lz_orth2", "lz_orth3", "splitk_reduce")
 
 
 def pmc_workload(QF, H, strategy, chi, direction):
     """Dispatches for the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE):
-    one chi x chi bond split in the plain form and one in the basis-sketched
-    form (the dominant svd_jacobi launches of each), then a plain copy of
-    PMC_COPY_ELEMS complex128 elements that calibrates the counters."""
+    one chi x chi bond update in the chosen form (eigensolver, basis-sketched
+    split, environment), a one-element marker copy, one chi x chi bond update
+    in the plain form, then a plain copy of PMC_COPY_ELEMS complex128 elements
+    that calibrates the counters."""
     import importlib
     import torch
-    from seemps2_amd import kernels as K
+    from seemps2_amd import kernels as K, learned
 
     D = importlib.import_module("seemps2_amd.optimization.dmrg")
     start = advance_to_cap(QF, strategy, direction, chi)
-    _, AB = D._diagonalize_two_site(QF, start, 1e-10)
-    a, d1, d2, b = AB.shape
-    theta = AB.reshape(a * d1, d2 * b)
-    m, n = theta.shape
-    if direction > 0:
-        basis, side, k = QF.state[start + 1], 0, QF.state[start + 1].shape[0]
-        bm = basis.reshape(k, n)
-    else:
-        basis, side, k = QF.state[start], 1, QF.state[start].shape[2]
-        bm = basis.reshape(m, k)
-    K.svd_split(theta, m, n, side, strategy.method, strategy.tolerance, chi)
-    K.svd_split_basis(theta, m, n, side, bm, k, 16, side, strategy.method, strategy.tolerance,
-                      chi, seed=3)
+    sites = [start, start + 1] if direction > 0 else [start, start - 1]
+    lt = learned.LearnedTruncation(ConstantPredictor(0.0), oversample=16, margin=1.25,
+                                   min_size=64)
+    for form, i in zip(("chosen", "plain"), sites):
+        _, AB = D._diagonalize_two_site(QF, i, 1e-10)
+        if form == "chosen":
+            with learned.learned_truncation(lt):
+                (QF.update_2site_right if direction > 0 else QF.update_2site_left)(AB, i, strategy)
+            m = K.empty(1)
+            K.fill_zero(m)
+            K.scale2d(m.view(1, 1), 1, 1, 1, out=K.empty(1, 1))  # marker between the forms
+        else:
+            (QF.update_2site_right if direction > 0 else QF.update_2site_left)(AB, i, strategy)
     x = K.empty(PMC_COPY_ELEMS)
     y = K.empty(PMC_COPY_ELEMS)
     K.fill_zero(x)
     K.scale2d(x.view(4096, 4096), 4096, 4096, 4096, out=y.view(4096, 4096))
     torch.cuda.synchronize()
-    log(f"pmc workload: split of bond {start} ({m} x {n}), basis side {side}, k = {k}")
+    log(f"pmc workload: bond updates at sites {sites} (chosen, then plain), "
+        f"{lt.basis_splits} basis-sketched split(s)")
 
 
 def pmc_summary(fetch_dir, write_dir):
-    """Traffic of the svd_jacobi launches of --pmc-workload from the two passes,
-    each counter scaled by what it reads for the calibration copy."""
+    """HBM bytes per dispatch of the main kernels of each form of
+    --pmc-workload, from the two passes, each counter scaled by what it reads
+    for the calibration copy."""
     import csv
     import glob
 
@@ -178,23 +188,40 @@ def pmc_summary(fetch_dir, write_dir):
     copy_kib = PMC_COPY_ELEMS * 16 / 1024
     cf = copy_kib / [v for (_, k), v in f if k == "zscale2d_kernel"][-1]
     cw = copy_kib / [v for (_, k), v in w if k == "zscale2d_kernel"][-1]
-    jf = [v for (_, k), v in f if k == "svd_jacobi"][-2:]
-    jw = [v for (_, k), v in w if k == "svd_jacobi"][-2:]
+
+    def split_forms(rows):
+        # the last two zscale2d dispatches are the marker and the calibration copy
+        marks = [i for i, ((_, k), _) in enumerate(rows) if k == "zscale2d_kernel"][-2:]
+        return {"chosen": rows[:marks[0]], "plain": rows[marks[0] + 1:marks[1]]}
+
+    ff, ww = split_forms(f), split_forms(w)
     forms = {}
-    for name, fk, wk in zip(("plain", "chosen"), jf, jw):
-        forms[name] = {"fetch_bytes": round(fk * cf * 1024), "write_bytes": round(wk * cw * 1024),
-                       "traffic_bytes": round((fk * cf + wk * cw) * 1024)}
-    out = {"kernel": "svd_jacobi", "fetch_scale": round(cf, 4), "write_scale": round(cw, 4),
-           "forms": forms}
-    return out
+    for form in ("chosen", "plain"):
+        kern = {}
+        for name in PMC_KERNELS:
+            fv = [v for (_, k), v in ff[form] if k == name]
+            wv = [v for (_, k), v in ww[form] if k == name]
+            if not fv or len(fv) != len(wv):
+                continue
+            fb = sum(fv) * cf * 1024 / len(fv)
+            wb = sum(wv) * cw * 1024 / len(wv)
+            kern[name] = {"dispatches": len(fv), "fetch_bytes": round(fb), "write_bytes": round(wb),
+                          "traffic_bytes": round(fb + wb)}
+        forms[form] = kern
+    return {"fetch_scale": round(cf, 4), "write_scale": round(cw, 4), "forms": forms}
 
 
-def read_traffic(form):
-    """HBM bytes per svd_jacobi launch of `form` from the committed PMC summary."""
+# library timing names whose launches appear under another name in rocprof
+ROCPROF_NAMES = {"polar_svd": "polar_persistent"}
+
+
+def read_traffic(kernel, form):
+    """HBM bytes per `kernel` dispatch of `form` from the committed PMC summary."""
+    kernel = ROCPROF_NAMES.get(kernel, kernel)
     try:
         text = open(PMC_PROFILE).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)["forms"][form]["traffic_bytes"]
+        return json.loads(block)["forms"][form][kernel]["traffic_bytes"]
     except (OSError, IndexError, KeyError, ValueError):
         return None
 
@@ -275,26 +302,45 @@ def timed_sweeps(QF, strategy, direction, steps, chi, precision="fp64"):
     return (time.perf_counter() - t0, t_chi, n_chi, n_all, D.STATS.matvecs, direction, E)
 
 
-# HIP-event timing inside the timed sweeps: the roofline kernel only, so the
-# events perturb the measured rate as little as possible (--kernel-timing
-# times more, e.g. "svd_jacobi,zgemm_kernel,zgemm_kernel_fp32" or "*")
-TIMED_KERNELS = "svd_jacobi"
+# HIP-event timing inside the timed sweeps: the candidates for the dominant
+# kernel of either form (the complex GEMM, the rotation-iteration SVD of the
+# sketched splits, the block Jacobi SVD of the full splits); --kernel-timing
+# times others, e.g. "*" for every instrumented launch
+TIMED_KERNELS = "zgemm_kernel,polar_svd,svd_jacobi"
 METRIC = "two-site bond updates/s at chi=256; speedup vs plain FP64 GPU at <=1e-9 rel err"
 
 
+class ConstantPredictor:
+    """Stand-in for the rank predictor that always answers the same kept
+    fraction (the ablation of the learned part: a fixed sketch width)."""
+
+    def __init__(self, frac):
+        self.frac = float(frac)
+
+    def predict(self, X):
+        import torch
+
+        return torch.full((X.shape[0],), self.frac, dtype=torch.float32, device=X.device)
+
+
 def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_plain,
-                 power_iters=1, precision="mixed", kernel_timing=TIMED_KERNELS):
+                 power_iters=1, precision="mixed", kernel_timing=TIMED_KERNELS,
+                 sweep_rate_plain=None, predictor=None, label="learned form"):
     """The same timed sweeps from the same state with the learned-rank split:
     predictor trained on the warm-up's spectra (device kernels), sketched
     randomized SVDs with full-SVD fallback, and the eigensolver's contractions in
     `precision`.  Returns the trained predictor and a record with the rate, the
-    speedup over the plain FP64 path and the energy difference."""
+    speedup over the plain FP64 path and the energy difference.  A given
+    `predictor` (e.g. a ConstantPredictor) is used as it is, untrained."""
     from seemps2_amd import learned
 
     X, t = spectra.tensors()
-    net = learned.RankPredictor(seed=11)
     t0 = time.perf_counter()
-    losses = net.train(X, t, epochs=400, lr=3e-3)
+    if predictor is None:
+        net = learned.RankPredictor(seed=11)
+        losses = net.train(X, t, epochs=400, lr=3e-3)
+    else:
+        net, losses = predictor, [float("nan")]
     t_train = time.perf_counter() - t0
     lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=power_iters,
                                    min_size=64)
@@ -313,7 +359,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     mv32 = D.STATS.matvecs_fp32
     value = n_chi / t_chi if t_chi > 0 else 0.0
     rel = abs(E - E_plain) / max(abs(E_plain), 1e-300)
-    log(f"learned form: {n_chi} chi={chi} updates in {t_chi:.3f} s ({value:.2f}/s, "
+    log(f"{label}: {n_chi} chi={chi} updates in {t_chi:.3f} s ({value:.2f}/s, "
         f"x{value / max(value_plain, 1e-30):.2f}), energy rel. diff {rel:.2e}, "
         f"{lt.fallbacks} fallbacks of {lt.calls} splits ({lt.basis_splits} basis-sketched), "
         f"{mv32}/{matvecs} FP32 matvecs, "
@@ -322,8 +368,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
-        if roof["kernel"] == "svd_jacobi":
-            roof["traffic"] = read_traffic("chosen")
+        roof["traffic"] = read_traffic(roof["kernel"], "chosen")
     return net, dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
                 bond_updates_at_chi=n_chi, seconds_in_chi_updates=round(t_chi, 4),
                 bond_updates_total=n_all, matvecs_per_update=round(matvecs / max(n_all, 1), 2),
@@ -334,7 +379,37 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
                 power_iters=power_iters, eigensolver_precision=precision,
                 matvecs=matvecs, matvecs_fp32=mv32,
                 training_spectra=int(X.shape[0]), training_seconds=round(t_train, 3),
-                ms_per_step=round(secs / max(steps, 1) * 1e3, 3))
+                ms_per_step=round(secs / max(steps, 1) * 1e3, 3),
+                sweep_bond_updates_per_s=round(n_all / secs, 3) if secs > 0 else 0.0,
+                whole_sweep_speedup_vs_plain=(round((n_all / secs) / sweep_rate_plain, 3)
+                                              if sweep_rate_plain and secs > 0 else None))
+
+
+def dmrg_dropin(state, H, strategy, sweeps=2, lt=None, precision="fp64"):
+    """The public dmrg() from a converged state, `sweeps` sweeps: the driver
+    as a user calls it, with its environment set-up and the energy / variance
+    pass after every sweep (the unsimplified H psi at bond w chi).  Returns
+    the wall time per sweep and the energy."""
+    import torch
+    import seemps2_amd as S
+    from seemps2_amd import kernels as K, learned
+
+    guess = S.CanonicalMPS([K.copy(A) for A in state], center=state.center, is_canonical=True,
+                           strategy=strategy)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if lt is not None:
+        with learned.learned_truncation(lt):
+            res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, strategy=strategy,
+                         precision=precision)
+    else:
+        res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, strategy=strategy,
+                     precision=precision)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    done = max(len(res.trajectory) - 1, 1)
+    return dict(seconds=round(secs, 4), sweeps=done, ms_per_sweep=round(secs / done * 1e3, 3),
+                energy=float(res.energy))
 
 
 def simplify_form(QF, H, chi, net, sweeps=2):
@@ -452,6 +527,74 @@ def batch_form(pairs=512, seed=7):
                 grouped_launches=bo.launches(), max_rel_diff=float(f"{rel:.2e}"))
 
 
+def many_simplify_form(net, pairs=512, seconds=20.0, seed=9):
+    """Simplify of many uneven sums psi_k + phi_k (N ~ U[10, 40] sites, chi ~
+    U[8, 128], half of them complex64) to tolerance 1e-8, state by state, in
+    the plain form and with the learned-rank basis-sketched splits, on a time-
+    bounded sample of the pairs.  Returns states/s both ways and the worst
+    fidelity with the exact sum."""
+    import numpy as np
+    import torch
+    import seemps2_amd as S
+    from seemps2_amd import learned
+
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+
+    def state(N, chi, dtype):
+        dims = [min(chi, 2 ** min(i, N - i)) for i in range(N + 1)]
+        return S.MPS([(torch.randn(dims[i], 2, dims[i + 1], dtype=torch.complex128,
+                                   device="cuda", generator=gen) / np.sqrt(2 * dims[i]))
+                      .to(dtype).to(torch.complex128) for i in range(N)])
+
+    specs = []
+    for k in range(pairs):
+        N = int(rng.integers(10, 41))
+        specs.append((N, int(rng.integers(8, 129)), int(rng.integers(8, 129)),
+                      torch.complex64 if k % 2 else torch.complex128))
+    st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+
+    def run(lt, budget):
+        done, worst, t_used = 0, 1.0, 0.0
+        for N, ca, cb, dt in specs:
+            a, b = state(N, ca, dt), state(N, cb, dt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if lt is None:
+                out = S.simplify(S.MPSSum([1.0, 1.0], [a, b]), strategy=st)
+            else:
+                with learned.learned_truncation(lt):
+                    out = S.simplify(S.MPSSum([1.0, 1.0], [a, b]), strategy=st)
+            torch.cuda.synchronize()
+            t_used += time.perf_counter() - t0
+            done += 1
+            ov = complex(S.scprod(out, a)) + complex(S.scprod(out, b))
+            n_out = abs(complex(S.scprod(out, out)))
+            n_sum = abs(complex(S.scprod(a, a)) + complex(S.scprod(b, b))
+                        + 2 * complex(S.scprod(a, b)).real)
+            worst = min(worst, abs(ov) ** 2 / (n_out * n_sum))
+            if t_used > budget:
+                break
+        return done, t_used, worst
+
+    n_p, t_p, f_p = run(None, seconds / 2)
+    lt = learned.LearnedTruncation(net, oversample=16, margin=1.25, power_iters=1, min_size=64)
+    # the same pairs in the chosen form
+    specs = specs[:n_p]
+    n_c, t_c, f_c = run(lt, 1e30)
+    log(f"many-state simplify: {n_p} uneven sums, plain {n_p / t_p:.2f} states/s, chosen "
+        f"{n_c / t_c:.2f} states/s (x{(n_c / t_c) / (n_p / t_p):.2f}), worst fidelity "
+        f"{f_p:.15f} / {f_c:.15f}, {lt.basis_splits} basis-sketched, {lt.fallbacks} fallbacks "
+        f"of {lt.calls} splits")
+    return dict(workload=f"simplify(psi_k + phi_k) state by state, N~U[10,40], chi~U[8,128], "
+                         f"half complex64, tolerance 1e-8; a time-bounded sample of {n_p} of "
+                         f"{pairs} pairs", states=n_p,
+                states_per_s_plain=round(n_p / t_p, 3), states_per_s_chosen=round(n_c / t_c, 3),
+                speedup_vs_plain=round((n_c / t_c) / (n_p / t_p), 3),
+                worst_fidelity_plain=float(f_p), worst_fidelity_chosen=float(f_c),
+                basis_splits=lt.basis_splits, fallbacks=lt.fallbacks, splits=lt.calls)
+
+
 def report_kernels(records, secs):
     for name, r in sorted(records.items(), key=lambda kv: -kv[1]["ms"]):
         log(f"kernel {name}: {r['launches']} launches, {r['ms']:.2f} ms "
@@ -461,9 +604,11 @@ def report_kernels(records, secs):
 
 def roofline(records):
     """Roofline of the dominant kernel from the in-region HIP-event records."""
-    if not records:
+    # phase_* records bracket several launches and carry no flops
+    real = {k: v for k, v in records.items() if v["flops"] > 0 and not k.startswith("phase_")}
+    if not real:
         return None
-    name, r = max(records.items(), key=lambda kv: kv[1]["ms"])
+    name, r = max(real.items(), key=lambda kv: kv[1]["ms"])
     secs = r["ms"] * 1e-3
     achieved = r["flops"] / secs / 1e12 if secs > 0 else 0.0
     return dict(bound="mfma", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
@@ -534,13 +679,38 @@ def cpu_baseline(QF, chi, budget_s, direction, start):
         log(f"cpu baseline: bond {i} updated in {dt:.2f} s ({count[0]} matvecs)")
         if t_chi > budget_s or time.perf_counter() - t_start > 2 * budget_s:
             break
+    # the same kind of update with one BLAS thread (the next bond of the sweep)
+    one = None
+    nxt = i + 1 if direction > 0 else i - 1
+    if (n_chi and 0 <= nxt < size - 1 and st.tensors[nxt].shape[0] == chi
+            and st.tensors[nxt + 1].shape[2] == chi):
+        try:
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(limits=1, user_api="blas"):
+                log(f"cpu baseline: bond {nxt} with one BLAS thread")
+                t0 = time.perf_counter()
+                count[0] = 0
+                t_upd[0] = t0
+                _, AB = oa.diagonalize_two_site(q, nxt, 1e-10)
+                if direction > 0:
+                    q.update_2site_right(AB, nxt, strategy)
+                else:
+                    q.update_2site_left(AB, nxt, strategy)
+                dt1 = time.perf_counter() - t0
+            one = dict(value=1.0 / dt1, seconds=round(dt1, 3), matvecs=count[0])
+            log(f"cpu baseline: one-thread update {dt1:.2f} s ({count[0]} matvecs)")
+        except Exception as e:  # pragma: no cover
+            log(f"cpu baseline: no one-thread sample ({e})")
     oa.two_site_matvec = matvec
     if n_chi == 0:
         return None
     return dict(value=n_chi / t_chi, unit="bond_updates/s", cores=int(cores), kind="port",
                 sample=f"{n_chi} oracle bond updates at chi={chi} x {chi} bonds (scipy eigsh "
-                       f"ncv=20 tol 1e-10, {n_mv} matvecs; numpy SVD) continuing the sweep on the "
-                       f"GPU-prepared state, {t_chi:.1f} s")
+                       f"ncv=20 tol 1e-10, {n_mv} matvecs, i.e. {n_mv / n_chi:.1f} per update; "
+                       f"numpy SVD) continuing the sweep on the GPU-prepared state, "
+                       f"{t_chi:.1f} s",
+                one_thread=one)
 
 
 def main():
@@ -585,6 +755,7 @@ def main():
             tw, *_rest, direction, _ = timed_sweeps(QF, strategy, direction, args.warmup, chi)
         log(f"warmup {args.warmup} sweep(s): {tw:.2f} s, {len(spectra.targets)} spectra logged")
     snapshot = copy.deepcopy(QF) if not args.no_learned else None
+    pristine = copy.deepcopy(QF) if (not args.no_learned and not args.no_ablation) else None
     direction0 = direction
     from seemps2_amd import _lib
 
@@ -601,17 +772,59 @@ def main():
     roof = roofline(records)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
-        if roof["kernel"] == "svd_jacobi":
-            roof["traffic"] = read_traffic("plain")
+        roof["traffic"] = read_traffic(roof["kernel"], "plain")
+    sweep_rate_plain = n_all / secs if secs > 0 else None
     chosen = net = None
     if snapshot is not None and len(spectra.targets) >= 16:
         net, chosen = learned_form(snapshot, strategy, direction0, args.steps, chi, spectra, value,
                                    E, args.sketch_power_iters, args.eig_precision,
-                                   args.kernel_timing)
+                                   args.kernel_timing, sweep_rate_plain=sweep_rate_plain)
+    # Ablation of the learned part: the same timed sweeps with the predictor
+    # replaced by constant answers, i.e. a fixed number of expansion
+    # directions (the basis-sketched splits at capped bonds never consult it)
+    ablation = None
+    if chosen is not None and pristine is not None:
+        from seemps2_amd import learned as _learned
+
+        ablation = []
+        for frac, label in ((0.0, "fixed 16 expansion directions"),
+                            (1.0, f"fixed {_learned.MAX_EXPANSION} expansion directions "
+                                  "(the cap)")):
+            _, rec = learned_form(copy.deepcopy(pristine), strategy, direction0, args.steps, chi,
+                                  spectra, value, E, args.sketch_power_iters,
+                                  args.eig_precision, "", sweep_rate_plain=sweep_rate_plain,
+                                  predictor=ConstantPredictor(frac), label=f"ablation ({label})")
+            ablation.append(dict(predictor=label, value=rec["value"],
+                                 speedup_vs_plain=rec["speedup_vs_plain"],
+                                 sweep_bond_updates_per_s=rec["sweep_bond_updates_per_s"],
+                                 fallbacks=rec["fallbacks"], splits=rec["splits"],
+                                 mean_sketch_width=rec["mean_sketch_width"],
+                                 energy_rel_diff=rec["energy_rel_diff"]))
+        del pristine
+    dropin = None
+    if chosen is not None and not args.no_dropin:
+        from seemps2_amd import learned as _learned
+
+        st_d = S.DEFAULT_STRATEGY.replace(max_bond_dimension=chi)
+        d_plain = dmrg_dropin(QF.state, H, st_d)
+        lt = _learned.LearnedTruncation(net, oversample=16, margin=1.25,
+                                        power_iters=args.sketch_power_iters, min_size=64)
+        d_chosen = dmrg_dropin(QF.state, H, st_d, lt=lt, precision=args.eig_precision)
+        dropin = dict(workload=f"seemps2_amd.dmrg(H, guess, maxiter=2) from the chi={chi} state, "
+                               "energy and variance of the unsimplified H psi after each sweep",
+                      plain=d_plain, chosen=d_chosen,
+                      speedup_vs_plain=round(d_plain["ms_per_sweep"] / d_chosen["ms_per_sweep"], 3),
+                      energy_rel_diff=float(f"{abs(d_chosen['energy'] - d_plain['energy']) / abs(d_plain['energy']):.3e}"))
+        log(f"dmrg() drop-in: plain {d_plain['ms_per_sweep']:.1f} ms/sweep, chosen "
+            f"{d_chosen['ms_per_sweep']:.1f} ms/sweep (x{dropin['speedup_vs_plain']:.2f}), "
+            f"energy rel. diff {dropin['energy_rel_diff']:.1e}")
     simp = None
     if chosen is not None and not args.no_simplify:
         simp = simplify_form(QF, H, chi, net)
     batched = None if args.no_batch else batch_form()
+    many = None
+    if net is not None and not args.no_batch:
+        many = many_simplify_form(net)
     cpu = None
     if not args.no_cpu_baseline:
         start = advance_to_cap(QF, strategy, direction, chi)
@@ -649,7 +862,9 @@ def main():
         "config": {
             "workload": f"two-site DMRG sweeps, XXZ chain N={N}, d=2, w=5, chi={chi}, "
                         "Krylov 20, tol 1e-10",
-            "form": ("mixed-precision eigensolver + learned-rank basis-sketched split"
+            "form": ("learned-rank basis-sketched split (FP64 rotation-iteration SVD) + "
+                     "FP64 Lanczos (its mixed-precision mode ran "
+                     f"{chosen['matvecs_fp32']} FP32 matvecs: warm starts stay in FP64)"
                      if use_chosen else "plain FP64"),
             "speedup_vs_plain_fp64": chosen["speedup_vs_plain"] if use_chosen else 1.0,
             "energy_rel_diff_vs_plain_fp64": chosen["energy_rel_diff"] if use_chosen else 0.0,
@@ -664,7 +879,10 @@ def main():
         "plain_fp64": plain,
         "chosen_form": chosen,
         "simplify_at_chi": simp,
+        "dmrg_dropin": dropin,
+        "ablation": ablation,
         "batched_overlaps": batched,
+        "many_state_simplify": many,
     }
     print(json.dumps(line), flush=True)
 

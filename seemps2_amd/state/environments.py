@@ -61,6 +61,27 @@ def update_right_mpo_environment(rho, A, O, B):
     return K.gemm(T2.view(a * c, i * f), B.reshape(b, i * f), opB=OP_T).view(a, c, b)
 
 
+def begin_mpo2_environment():
+    return K.as_dev(np.ones((1, 1, 1, 1)))
+
+
+def update_left_mpo2_environment(E, A, O):
+    """E'[a',w1',w2',b'] = sum E[a,w1,w2,b] conj(A[a,i,a']) conj(O[w1,k,i,w1'])
+    O[w2,k,j,w2'] A[b,j,b']: the left environment of <O psi|O psi> with both
+    MPO layers (no counterpart in the reference, which applies O and folds the
+    MPS product of bond w chi).  Two GEMMs of chi^3 d w^2 complex MACs around
+    two small local contractions."""
+    E, A, O = K.as_dev(E), K.as_dev(A), K.as_dev(O)
+    a, w1, w2, b = E.shape
+    _, j, b2 = A.shape
+    c, k, i, e = O.shape
+    T1 = K.gemm(E.reshape(a * w1 * w2, b), A.reshape(b, j * b2))             # [a,w1,w2,j,b']
+    T2 = K.local(T1, _left_op(O), a * w1, w2 * j, b2, k * e, 1)               # [a,w1,k,w2',b']
+    T3 = K.local(T2, K.conj(O).view(c * k, i * e), a, w1 * k, e * b2, i * e, 1)  # [a,i,w1',w2',b']
+    return K.gemm(A.reshape(a * i, A.shape[2]), T3.view(a * i, e * e * b2),
+                  opA=OP_H).view(A.shape[2], e, e, b2)
+
+
 def end_mpo_environment(rho) -> complex:
     return complex(rho.reshape(-1)[0].item())
 
@@ -79,6 +100,8 @@ __all__ = [
     "scprod",
     "vdot",
     "begin_mpo_environment",
+    "begin_mpo2_environment",
+    "update_left_mpo2_environment",
     "update_left_mpo_environment",
     "update_right_mpo_environment",
     "end_mpo_environment",

@@ -222,3 +222,30 @@ def test_dmrg_mixed_precision_matches_fp64(S):
     assert abs(out.energy - ex.xx_free_fermion_energy(N)) < 1e-9
     with pytest.raises(ValueError):
         S.dmrg(H, dev_mps(S, g), maxiter=1, strategy=st, precision="fp16")
+
+
+def test_energy_and_variance_two_layer_environments(S):
+    """The DMRG driver's variance | |H psi|^2 - E^2 | (dmrg.py:55-59) from
+    environments that carry both MPO layers equals the norm of the explicit
+    H psi (the reference's route) and the oracle's value."""
+    import importlib
+
+    D = importlib.import_module("seemps2_amd.optimization.dmrg")
+    rng = np.random.default_rng(77)
+    N = 9
+    H = ob.xxz_mpo(N)
+    a = ob.random_mps([2] * N, D=7, complex=True, rng=rng)
+    psi = S.CanonicalMPS(dev_mps(S, a), center=0)
+    E, var = D._energy_and_variance(S.MPO(H), psi)
+    Hpsi = S.MPO(H).apply(psi, simplify=False)
+    n_explicit = Hpsi.norm_squared()
+    n_two_layer = D._applied_norm_squared(S.MPO(H), psi)
+    assert abs(n_two_layer - n_explicit) < 1e-12 * n_explicit
+    o = oa.canonical(a, 0, oc.DEFAULT_STRATEGY)
+    E_ref, var_ref = oa.energy_and_variance(H, o)
+    assert abs(E - E_ref) < 1e-12 * abs(E_ref)
+    assert abs(var - var_ref) < 1e-11 * max(abs(var_ref), 1.0)
+    # and against the dense operator
+    v = psi.to_vector()
+    Hd = ex.mpo_to_matrix(H)
+    assert abs(n_two_layer - np.vdot(Hd @ v, Hd @ v).real) < 1e-11 * n_two_layer

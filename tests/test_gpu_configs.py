@@ -403,3 +403,47 @@ def test_config5_haar_layers_learned_vs_full_n48_d4_chi512(S):
            fallback_rate=lt.fallbacks / max(sketched, 1))
     assert 1 - f < 1e-9
     assert d_err < 1e-9
+
+
+# --------------------------------------------------------------- workload 4
+
+
+def test_config4_many_uneven_sums(S):
+    """Random MPS pairs of uneven length (N ~ U[10, 40]) and bond dimension
+    (chi ~ U[8, 128]), half of them complex64, each sum simplified to
+    tolerance 1e-8.  Every result against the exact sum (fidelity from
+    overlaps), two of them against the oracle on the same inputs (bond
+    dimensions and states)."""
+    rng = np.random.default_rng(404)
+    st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+    ost = oc.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+    worst, checked = 1.0, 0
+    for k in range(12):
+        # the first two are small enough for the oracle's NumPy SVDs and a
+        # dense comparison
+        N = int(rng.integers(10, 41)) if k >= 2 else int(rng.integers(10, 17))
+        chi = int(rng.integers(8, 129)) if k >= 2 else int(rng.integers(8, 33))
+        a = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
+        b = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
+        if k % 2:
+            a = oa.OMPS([A.astype(np.complex64).astype(np.complex128) for A in a.tensors], 0.0)
+            da = S.MPS([A.astype(np.complex64) for A in a.tensors])
+        else:
+            da = S.MPS(a.tensors)
+        db = S.MPS(b.tensors)
+        out = S.simplify(S.MPSSum([1.0, 1.0], [da, db]), strategy=st)
+        ov = complex(S.scprod(out, da)) + complex(S.scprod(out, db))
+        n_out = abs(complex(S.scprod(out, out)))
+        n_sum = abs(complex(S.scprod(da, da)) + complex(S.scprod(db, db))
+                    + 2 * complex(S.scprod(da, db)).real)
+        fid = abs(ov) ** 2 / (n_out * n_sum)
+        worst = min(worst, fid)
+        if k < 2:
+            ref = oa.simplify(oa.OMPSSum([1.0, 1.0], [a, b]), ost)
+            assert out.bond_dimensions() == ref.bond_dimensions()
+            assert rel(out.to_vector(), ref.to_vector()) < 1e-10
+            checked += 1
+        report("config4 state", k=k, N=N, chi=chi, max_bond=out.max_bond_dimension(),
+               fidelity=fid)
+    report("config4", states=12, worst_fidelity=worst, oracle_checked=checked)
+    assert 1 - worst < 1e-8
