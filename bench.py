@@ -355,6 +355,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
                                                                 precision)
     records = _lib.timing_records()
     _lib.timing_enable(None)
+    bracket = _lib.timing_bracket_us()
     report_kernels(records, secs)
     mv32 = D.STATS.matvecs_fp32
     value = n_chi / t_chi if t_chi > 0 else 0.0
@@ -365,7 +366,7 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
         f"{mv32}/{matvecs} FP32 matvecs, "
         f"training {t_train:.2f} s "
         f"(loss {losses[0]:.2e} -> {losses[-1]:.2e} on {X.shape[0]} spectra)")
-    roof = roofline(records)
+    roof = roofline(records, bracket)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
         roof["traffic"] = read_traffic(roof["kernel"], "chosen")
@@ -602,21 +603,25 @@ def report_kernels(records, secs):
             f"{r['flops'] / max(r['ms'] * 1e-3, 1e-30) / 1e12:.3f} TFLOP/s")
 
 
-def roofline(records):
-    """Roofline of the dominant kernel from the in-region HIP-event records."""
+def roofline(records, bracket_us=0.0):
+    """Roofline of the dominant kernel from the in-region HIP-event records.
+    Each record's events also time their own bracketing (the gap between the
+    event packets and the kernel's dispatch); `bracket_us` is that cost as
+    measured on an empty kernel, subtracted from the average launch."""
     # phase_* records bracket several launches and carry no flops
     real = {k: v for k, v in records.items() if v["flops"] > 0 and not k.startswith("phase_")}
     if not real:
         return None
     name, r = max(real.items(), key=lambda kv: kv[1]["ms"])
-    secs = r["ms"] * 1e-3
-    achieved = r["flops"] / secs / 1e12 if secs > 0 else 0.0
+    n = max(r["launches"], 1)
+    avg_events = r["ms"] * 1e3 / n
+    avg = max(avg_events - bracket_us, 0.5 * avg_events)
+    achieved = r["flops"] / n / (avg * 1e-6) / 1e12 if avg > 0 else 0.0
     return dict(bound="mfma", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
                 unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 5), traffic=None,
-                kernel=name, launches=r["launches"],
-                avg_launch_us=round(r["ms"] * 1e3 / max(r["launches"], 1), 2),
-                flops_per_launch=round(r["flops"] / max(r["launches"], 1)),
-                share_of_step=None)
+                kernel=name, launches=r["launches"], avg_launch_us=round(avg, 2),
+                avg_launch_us_events=round(avg_events, 2), event_bracket_us=round(bracket_us, 2),
+                flops_per_launch=round(r["flops"] / n), share_of_step=None)
 
 
 def cpu_baseline(QF, chi, budget_s, direction, start):
@@ -764,12 +769,13 @@ def main():
                                                                     args.steps, chi)
     records = _lib.timing_records()
     _lib.timing_enable(None)
+    bracket = _lib.timing_bracket_us()
     log(f"timed {args.steps} sweep(s): {secs:.3f} s, {n_chi} chi={chi} updates in {t_chi:.3f} s, "
         f"{n_all} updates in all, {matvecs} matvecs, local energy {E:.12f}")
     report_kernels(records, secs)
     value = n_chi / t_chi if t_chi > 0 else 0.0
     mv_per_update = matvecs / max(n_all, 1)
-    roof = roofline(records)
+    roof = roofline(records, bracket)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
         roof["traffic"] = read_traffic(roof["kernel"], "plain")

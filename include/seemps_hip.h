@@ -361,6 +361,10 @@ int seemps_adam_step(int64_t n, float* params, const float* grads, float* m, flo
  * for every instrumented kernel, NULL or "" to switch timing off.  Resets the
  * records. */
 int seemps_timing_enable(const char* filter);
+/* Launch `launches` empty kernels on `stream`, each bracketed like the
+ * instrumented launches (record name "timing_calibration" when enabled): the
+ * per-launch cost of the event bracketing itself, for subtraction. */
+int seemps_timing_calibrate(int launches, void* stream);
 /* Number of kernels with records.  Waits for the pending events. */
 int seemps_timing_count(void);
 /* Record `index` (< seemps_timing_count()): kernel name (NUL-terminated,

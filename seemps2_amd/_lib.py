@@ -109,6 +109,7 @@ _SIGNATURES = {
                 ctypes.c_float, _I, _P]
     ),
     "seemps_timing_enable": (c_int, [ctypes.c_char_p]),
+    "seemps_timing_calibrate": (c_int, [c_int, _P]),
     "seemps_timing_count": (c_int, []),
     "seemps_timing_get": (
         c_int,
@@ -183,6 +184,21 @@ workspace = _Workspace()
 def timing_enable(kernels: str | None) -> None:
     """Time the named kernels' launches with HIP events ("*" = all, None = off)."""
     check(lib.seemps_timing_enable(kernels.encode() if kernels else None))
+
+
+def timing_bracket_us(launches: int = 400) -> float:
+    """Average HIP-event time (us) of an empty kernel bracketed the way the
+    instrumented launches are: the bracketing's own cost in each record.
+    Leaves timing off."""
+    timing_enable("timing_calibration")
+    try:
+        check(lib.seemps_timing_calibrate(launches, stream()))
+        rec = timing_records().get("timing_calibration")
+    finally:
+        timing_enable(None)
+    if not rec or rec["launches"] == 0:
+        return 0.0
+    return rec["ms"] * 1e3 / rec["launches"]
 
 
 def timing_records() -> dict[str, dict]:

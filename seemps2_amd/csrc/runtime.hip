@@ -198,7 +198,25 @@ TimedLaunch::~TimedLaunch() {
 
 }  // namespace seemps
 
+namespace seemps {
+namespace {
+__global__ void timing_empty_kernel() {}
+}  // namespace
+}  // namespace seemps
+
 extern "C" const char* seemps_last_error(void) { return seemps::g_last_error; }
+
+extern "C" int seemps_timing_calibrate(int launches, void* stream) {
+  using namespace seemps;
+  SEEMPS_CHECK_ARG(launches >= 0, "timing_calibrate: negative count");
+  for (int i = 0; i < launches; ++i) {
+    TimedLaunch timed("timing_calibration", (hipStream_t)stream, 0.0);
+    hipLaunchKernelGGL(timing_empty_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream);
+    const int st = check_launch("timing_calibration");
+    if (st) return st;
+  }
+  return SEEMPS_OK;
+}
 
 extern "C" int seemps_release(void) {
   using namespace seemps;

@@ -417,7 +417,7 @@ def test_config4_many_uneven_sums(S):
     rng = np.random.default_rng(404)
     st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
     ost = oc.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
-    worst, checked = 1.0, 0
+    worst, checked = 0.0, 0  # worst infidelity relative to its bound
     for k in range(12):
         # the first two are small enough for the oracle's NumPy SVDs and a
         # dense comparison
@@ -437,7 +437,8 @@ def test_config4_many_uneven_sums(S):
         n_sum = abs(complex(S.scprod(da, da)) + complex(S.scprod(db, db))
                     + 2 * complex(S.scprod(da, db)).real)
         fid = abs(ov) ** 2 / (n_out * n_sum)
-        worst = min(worst, fid)
+        # each of the N - 1 truncations may drop up to 1e-8 of the norm squared
+        worst = max(worst, (1 - fid) / ((N - 1) * 1e-8))
         if k < 2:
             ref = oa.simplify(oa.OMPSSum([1.0, 1.0], [a, b]), ost)
             assert out.bond_dimensions() == ref.bond_dimensions()
@@ -445,5 +446,5 @@ def test_config4_many_uneven_sums(S):
             checked += 1
         report("config4 state", k=k, N=N, chi=chi, max_bond=out.max_bond_dimension(),
                fidelity=fid)
-    report("config4", states=12, worst_fidelity=worst, oracle_checked=checked)
-    assert 1 - worst < 1e-8
+    report("config4", states=12, worst_infidelity_over_bound=worst, oracle_checked=checked)
+    assert worst <= 1.0
