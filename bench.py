@@ -23,6 +23,7 @@ import argparse
 import copy
 import json
 import os
+import re
 import sys
 import time
 
@@ -58,6 +59,8 @@ def parse():
                          "(the program profiled by the rocprofv3 --pmc passes)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce the two --pmc passes of --pmc-workload (no GPU work)")
+    ap.add_argument("--pmc-mfma", metavar="DIR",
+                    help="reduce an MFMA-counter --pmc pass of --pmc-workload (no GPU work)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-overlap measurement of many uneven states")
     ap.add_argument("--no-ablation", action="store_true",
@@ -122,8 +125,21 @@ def rocprof_summary(directory, bench_json=None, top=25):
                 avg = sum(last) / len(last) / 1e3
                 out.append("")
                 out.append(f"Timed region (last {len(last)} dispatches of `{kern}` in the "
-                           f"trace): average {avg:.1f} us; bench.py's HIP-event average "
-                           f"{roof.get('avg_launch_us')} us.")
+                           f"trace): average {avg:.1f} us.")
+                out.append(f"bench.py's HIP-event average of the same dispatches: "
+                           f"{roof.get('avg_launch_us_events')} us; event time of an empty "
+                           f"kernel {roof.get('event_bracket_us')} us; reported launch "
+                           f"average {roof.get('avg_launch_us')} us.")
+                empty = [r for r in rows if kernel_base_name(r["Name"]) == "timing_empty_kernel"]
+                if empty:
+                    e_us = float(empty[0]["AverageNs"]) / 1e3
+                    out.append(f"The empty calibration kernel itself: {empty[0]['Calls']} "
+                               f"dispatches, average {e_us:.2f} us.")
+                    ev, br = roof.get("avg_launch_us_events"), roof.get("event_bracket_us")
+                    if ev is not None and br is not None:
+                        out.append(f"HIP-event average less the event gaps ({br} - {e_us:.2f} "
+                                   f"us): {ev - (br - e_us):.2f} us against the trace's "
+                                   f"{avg:.1f} us.")
     return "\n".join(out)
 
 
@@ -135,30 +151,34 @@ PMC_KERNELS = ("zgemm_kernel", "polar_svd", "polar_persistent", "svd_jacobi", "z
 
 
 def pmc_workload(QF, H, strategy, chi, direction):
-    """Dispatches for the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE):
-    one chi x chi bond update in the chosen form (eigensolver, basis-sketched
-    split, environment), a one-element marker copy, one chi x chi bond update
-    in the plain form, then a plain copy of PMC_COPY_ELEMS complex128 elements
-    that calibrates the counters."""
+    """Dispatches for the rocprofv3 --pmc passes: an empty marker kernel, one
+    chi x chi bond update in the chosen form (eigensolver, basis-sketched
+    split, environment), a marker, one chi x chi bond update in the plain form,
+    a marker, then a plain copy of PMC_COPY_ELEMS complex128 elements that
+    calibrates the byte counters.  The marker is a kernel that nothing else
+    launches, so the passes' rows split into the two forms unambiguously."""
     import importlib
     import torch
-    from seemps2_amd import kernels as K, learned
+    from seemps2_amd import _lib, kernels as K, learned
 
     D = importlib.import_module("seemps2_amd.optimization.dmrg")
     start = advance_to_cap(QF, strategy, direction, chi)
     sites = [start, start + 1] if direction > 0 else [start, start - 1]
     lt = learned.LearnedTruncation(ConstantPredictor(0.0), oversample=16, margin=1.25,
                                    min_size=64)
+
+    def marker():
+        _lib.check(_lib.lib.seemps_timing_calibrate(1, _lib.stream()))
+
+    marker()
     for form, i in zip(("chosen", "plain"), sites):
         _, AB = D._diagonalize_two_site(QF, i, 1e-10)
         if form == "chosen":
             with learned.learned_truncation(lt):
                 (QF.update_2site_right if direction > 0 else QF.update_2site_left)(AB, i, strategy)
-            m = K.empty(1)
-            K.fill_zero(m)
-            K.scale2d(m.view(1, 1), 1, 1, 1, out=K.empty(1, 1))  # marker between the forms
         else:
             (QF.update_2site_right if direction > 0 else QF.update_2site_left)(AB, i, strategy)
+        marker()
     x = K.empty(PMC_COPY_ELEMS)
     y = K.empty(PMC_COPY_ELEMS)
     K.fill_zero(x)
@@ -168,39 +188,47 @@ def pmc_workload(QF, H, strategy, chi, direction):
         f"{lt.basis_splits} basis-sketched split(s)")
 
 
+PMC_MARKER = "timing_empty_kernel"
+
+
+def _pmc_rows(directory):
+    """[((dispatch id, kernel), {counter: value}, duration ns)] of a --pmc pass."""
+    import csv
+    import glob
+
+    path = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)[0]
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        key = (int(r["Dispatch_Id"]), kernel_base_name(r["Kernel_Name"]))
+        c, dur = acc.setdefault(key, ({}, int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+        c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [(k, c, d) for k, (c, d) in sorted(acc.items())]
+
+
+def _split_forms(rows):
+    """The chosen form's rows lie between the first two markers, the plain
+    form's between the second and the third."""
+    marks = [i for i, ((_, k), _, _) in enumerate(rows) if k == PMC_MARKER][-3:]
+    if len(marks) != 3:
+        raise SystemExit(f"expected 3 {PMC_MARKER} markers, found {len(marks)}")
+    return {"chosen": rows[marks[0] + 1:marks[1]], "plain": rows[marks[1] + 1:marks[2]]}
+
+
 def pmc_summary(fetch_dir, write_dir):
     """HBM bytes per dispatch of the main kernels of each form of
     --pmc-workload, from the two passes, each counter scaled by what it reads
     for the calibration copy."""
-    import csv
-    import glob
-
-    def per_dispatch(directory):
-        path = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
-                         recursive=True)[0]
-        acc = {}
-        for r in csv.DictReader(open(path)):
-            key = (int(r["Dispatch_Id"]), kernel_base_name(r["Kernel_Name"]))
-            acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
-        return sorted(acc.items())
-
-    f, w = per_dispatch(fetch_dir), per_dispatch(write_dir)
+    f, w = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
     copy_kib = PMC_COPY_ELEMS * 16 / 1024
-    cf = copy_kib / [v for (_, k), v in f if k == "zscale2d_kernel"][-1]
-    cw = copy_kib / [v for (_, k), v in w if k == "zscale2d_kernel"][-1]
-
-    def split_forms(rows):
-        # the last two zscale2d dispatches are the marker and the calibration copy
-        marks = [i for i, ((_, k), _) in enumerate(rows) if k == "zscale2d_kernel"][-2:]
-        return {"chosen": rows[:marks[0]], "plain": rows[marks[0] + 1:marks[1]]}
-
-    ff, ww = split_forms(f), split_forms(w)
+    cf = copy_kib / [c["FETCH_SIZE"] for (_, k), c, _ in f if k == "zscale2d_kernel"][-1]
+    cw = copy_kib / [c["WRITE_SIZE"] for (_, k), c, _ in w if k == "zscale2d_kernel"][-1]
+    ff, ww = _split_forms(f), _split_forms(w)
     forms = {}
     for form in ("chosen", "plain"):
         kern = {}
         for name in PMC_KERNELS:
-            fv = [v for (_, k), v in ff[form] if k == name]
-            wv = [v for (_, k), v in ww[form] if k == name]
+            fv = [c["FETCH_SIZE"] for (_, k), c, _ in ff[form] if k == name]
+            wv = [c["WRITE_SIZE"] for (_, k), c, _ in ww[form] if k == name]
             if not fv or len(fv) != len(wv):
                 continue
             fb = sum(fv) * cf * 1024 / len(fv)
@@ -211,19 +239,67 @@ def pmc_summary(fetch_dir, write_dir):
     return {"fetch_scale": round(cf, 4), "write_scale": round(cw, 4), "forms": forms}
 
 
+MI355X_CUS = 256
+MI355X_XCDS = 8
+
+
+def pmc_mfma(directory):
+    """FP64 MFMA work and matrix-core busy fraction per kernel of each form of
+    --pmc-workload, from a pass with SQ_INSTS_VALU_MFMA_MOPS_F64 (512 flops
+    each), SQ_VALU_MFMA_BUSY_CYCLES (summed over SIMDs) and GRBM_GUI_ACTIVE
+    (summed over the XCDs): busy = MFMA busy cycles / (GPU cycles x SIMDs)."""
+    rows = _split_forms(_pmc_rows(directory))
+    forms = {}
+    for form, rr in rows.items():
+        kern = {}
+        for name in PMC_KERNELS:
+            sel = [(c, d) for (_, k), c, d in rr if k == name]
+            if not sel:
+                continue
+            mops = sum(c.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) for c, _ in sel)
+            busy = sum(c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) for c, _ in sel)
+            gui = sum(c.get("GRBM_GUI_ACTIVE", 0.0) for c, _ in sel) / MI355X_XCDS
+            ns = sum(d for _, d in sel)
+            kern[name] = {"dispatches": len(sel), "fp64_mfma_flops": round(mops * 512),
+                          "mfma_tflops_during_kernel": round(mops * 512 / ns / 1e3, 3) if ns else None,
+                          "mfma_busy_frac": round(busy / (gui * MI355X_CUS * 4), 4) if gui else None,
+                          "gpu_cycles_per_ns": round(gui / ns, 3) if ns else None}
+        forms[form] = kern
+    return {"forms": forms}
+
+
 # library timing names whose launches appear under another name in rocprof
 ROCPROF_NAMES = {"polar_svd": "polar_persistent"}
 
 
+STATS_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                             "dmrg_xxz_n64_chi256_kernel_stats.md")
+
+
+def read_empty_dispatch_us():
+    """rocprofv3's duration of the empty calibration kernel (us) from the
+    committed kernel-statistics summary: the fixed cost of a one-workgroup
+    dispatch, which the HIP-event bracketing of an empty kernel includes and
+    every kernel's own duration includes as well.  None if not recorded."""
+    try:
+        m = re.search(r"empty calibration kernel itself: \d+ dispatches, average ([0-9.]+) us",
+                      open(STATS_PROFILE).read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
+
+
 def read_traffic(kernel, form):
-    """HBM bytes per `kernel` dispatch of `form` from the committed PMC summary."""
+    """(HBM bytes, MFMA flops) per `kernel` dispatch of `form` in the committed
+    PMC summary of one chi x chi bond update, or (None, None)."""
     kernel = ROCPROF_NAMES.get(kernel, kernel)
     try:
         text = open(PMC_PROFILE).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)["forms"][form][kernel]["traffic_bytes"]
+        rec = json.loads(block)["forms"][form][kernel]
+        return rec["traffic_bytes"], rec.get("fp64_mfma_flops")
     except (OSError, IndexError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def grow(QF, strategy, chi, log_every=True, max_sweeps=12):
@@ -369,7 +445,9 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
     roof = roofline(records, bracket)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
-        roof["traffic"] = read_traffic(roof["kernel"], "chosen")
+        # traffic of the same kernel in the profiled bond update, whose
+        # dispatches differ in size from the timed sweeps' average
+        roof["traffic"], roof["traffic_dispatch_flops"] = read_traffic(roof["kernel"], "chosen")
     return net, dict(value=round(value, 3), unit="bond_updates/s", roofline=roof,
                 bond_updates_at_chi=n_chi, seconds_in_chi_updates=round(t_chi, 4),
                 bond_updates_total=n_all, matvecs_per_update=round(matvecs / max(n_all, 1), 2),
@@ -605,9 +683,9 @@ def report_kernels(records, secs):
 
 def roofline(records, bracket_us=0.0):
     """Roofline of the dominant kernel from the in-region HIP-event records.
-    Each record's events also time their own bracketing (the gap between the
-    event packets and the kernel's dispatch); `bracket_us` is that cost as
-    measured on an empty kernel, subtracted from the average launch."""
+    Each record's events also time their own bracketing (the gaps between the
+    event packets and the kernel's dispatch); `bracket_us` is the event time of
+    an empty kernel, which is those gaps plus the empty kernel's own duration."""
     # phase_* records bracket several launches and carry no flops
     real = {k: v for k, v in records.items() if v["flops"] > 0 and not k.startswith("phase_")}
     if not real:
@@ -615,12 +693,17 @@ def roofline(records, bracket_us=0.0):
     name, r = max(real.items(), key=lambda kv: kv[1]["ms"])
     n = max(r["launches"], 1)
     avg_events = r["ms"] * 1e3 / n
-    avg = max(avg_events - bracket_us, 0.5 * avg_events)
+    # the empty kernel's bracket holds the event gaps plus its own dispatch
+    # cost; only the gaps are removed (none if that cost is not on record)
+    empty = read_empty_dispatch_us()
+    gaps = max(bracket_us - empty, 0.0) if empty is not None else 0.0
+    avg = max(avg_events - gaps, 0.5 * avg_events)
     achieved = r["flops"] / n / (avg * 1e-6) / 1e12 if avg > 0 else 0.0
     return dict(bound="mfma", achieved=round(achieved, 4), peak=PEAK_FP64_TFLOPS,
                 unit="TFLOP/s", frac=round(achieved / PEAK_FP64_TFLOPS, 5), traffic=None,
                 kernel=name, launches=r["launches"], avg_launch_us=round(avg, 2),
                 avg_launch_us_events=round(avg_events, 2), event_bracket_us=round(bracket_us, 2),
+                empty_dispatch_us=empty,
                 flops_per_launch=round(r["flops"] / n), share_of_step=None)
 
 
@@ -726,6 +809,9 @@ def main():
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
         return
+    if args.pmc_mfma:
+        print(json.dumps(pmc_mfma(args.pmc_mfma), indent=1))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the DMRG sweep is "
               "sequential along the chain and this build measures one MI355X.", file=sys.stderr)
@@ -778,7 +864,9 @@ def main():
     roof = roofline(records, bracket)
     if roof is not None:
         roof["share_of_step"] = round(records[roof["kernel"]]["ms"] * 1e-3 / secs, 4)
-        roof["traffic"] = read_traffic(roof["kernel"], "plain")
+        # traffic of the same kernel in the profiled bond update, whose
+        # dispatches differ in size from the timed sweeps' average
+        roof["traffic"], roof["traffic_dispatch_flops"] = read_traffic(roof["kernel"], "plain")
     sweep_rate_plain = n_all / secs if secs > 0 else None
     chosen = net = None
     if snapshot is not None and len(spectra.targets) >= 16:
