@@ -42,6 +42,21 @@ def test_zgemm_ops(K, rng, opA, opB, M, N, K_):
     assert rel(K.to_numpy(out), ref) < 1e-14
 
 
+@pytest.mark.parametrize("opA,opB", [(0, 0), (2, 3), (1, 2), (3, 1)])
+def test_zgemm_large_tile_path(K, rng, opA, opB):
+    """Outputs with at least four 64 x 64 tiles per CU take the 64 x 64 kernel
+    (smaller ones the 32 x 32 kernel): ragged M, N and K on that path."""
+    M, N, K_ = 2048 + 17, 2048 + 40, 37
+    assert (M // 64 + 1) * (N // 64 + 1) >= 4 * torch.cuda.get_device_properties(0).multi_processor_count
+    A = crandn(rng, *((K_, M) if opA in (1, 2) else (M, K_)))
+    B = crandn(rng, *((N, K_) if opB in (1, 2) else (K_, N)))
+    C = crandn(rng, M, N)
+    alpha, beta = -0.4 + 0.9j, 0.25 - 0.5j
+    ref = alpha * OPS[opA](A) @ OPS[opB](B) + beta * C
+    out = K.gemm(K.as_dev(A), K.as_dev(B), opA, opB, alpha, beta, K.as_dev(C))
+    assert rel(K.to_numpy(out), ref) < 1e-14
+
+
 @pytest.mark.parametrize("opA,opB", [(0, 0), (2, 0), (0, 1), (3, 2), (1, 3)])
 @pytest.mark.parametrize("M,N,K_", [(64, 48, 2048), (512, 512, 1280), (5, 1, 9000)])
 def test_zgemm_splitk_through_python(K, rng, opA, opB, M, N, K_):
