@@ -28,6 +28,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "jacobi_inner.h"
@@ -36,7 +38,22 @@ namespace seemps {
 
 namespace {
 
+// SEEMPS_POLAR_TRACE: 1 prints a summary per call, 2 also the barrier
+// timings (diagnostics; both synchronise)
+int trace_level() {
+  static const int lvl = [] {
+    const char* e = getenv("SEEMPS_POLAR_TRACE");
+    return e ? atoi(e) : 0;
+  }();
+  return lvl;
+}
+long long* g_trace_buf = nullptr;
+
 constexpr int POLAR_MAX_ITERS = 12;
+// Newton-Schulz steps beyond this (a rotation far from the identity: a matrix
+// whose columns are not nearly orthogonal) end the iteration as not
+// converged, and the caller takes the block Jacobi.
+constexpr int POLAR_MAX_NS = 10;
 // An iteration that starts below this normalised off-diagonal leaves ~1e-18
 // (quadratic convergence), so it is the last one.
 constexpr double POLAR_QUAD_STOP = 1e-9;
@@ -177,9 +194,9 @@ __global__ void polar_output(int64_t Lg, int64_t l, const double2* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // The whole iteration in one cooperative kernel: the GEMMs are spread over
-// all workgroups as 16 x 16 output tiles (one wavefront each, FP64 MFMA),
-// separated by grid barriers, and the convergence test is read on the device,
-// so no launch or host round trip sits between the small products.
+// all workgroups as 16 x 16 output tiles (FP64 MFMA), separated by grid
+// barriers, and the convergence test is read on the device, so no launch or
+// host round trip sits between the small products.
 
 typedef double pd4 __attribute__((ext_vector_type(4)));
 
@@ -190,7 +207,10 @@ struct PolarCtrl {
   int iterations;
   int converged;
   unsigned long long stats[2][2];  // per iteration parity: max off, |X| bound (bits)
-  long long prof[4];  // workgroup 0: ticks waiting in barriers, ticks in all, barriers
+  // workgroup 0: [0] ticks waiting in barriers, [1] ticks in all, [2] barriers,
+  // [3] Newton-Schulz steps, ticks in [4] Gram, [5] generator, [6] Newton-Schulz,
+  // [7] rotation phases (each with its closing barrier)
+  long long prof[8];
 };
 
 constexpr unsigned long long POLAR_SPIN_LIMIT = 4000000ull;
@@ -199,12 +219,19 @@ constexpr unsigned long long POLAR_SPIN_LIMIT = 4000000ull;
 // last workgroup to arrive publishes the phase in a separate word, which the
 // others poll, so the waiters do not contend with the arrivals.  Returns false
 // once any workgroup has given up waiting.
-__device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok) {
+// `trace` (diagnostics, normally null): arrival time of every workgroup at
+// each of the first POLAR_TRACE_BARRIERS barriers, then workgroup 0's
+// departure times.
+constexpr int POLAR_TRACE_BARRIERS = 64;
+__device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok,
+                              long long* trace) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned int ph = ++*phase;
     const unsigned int target = ph * gridDim.x;
     const long long t0 = (blockIdx.x == 0) ? (long long)wall_clock64() : 0;
+    if (trace && ph <= POLAR_TRACE_BARRIERS)
+      trace[(ph - 1) * gridDim.x + blockIdx.x] = (long long)wall_clock64();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int before = atomicAdd(&c->barrier, 1u);
     int ok = 1;
@@ -227,8 +254,11 @@ __device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (blockIdx.x == 0) {
-      c->prof[0] += (long long)wall_clock64() - t0;
+      const long long t1 = (long long)wall_clock64();
+      c->prof[0] += t1 - t0;
       c->prof[2] += 1;
+      if (trace && ph <= POLAR_TRACE_BARRIERS)
+        trace[POLAR_TRACE_BARRIERS * gridDim.x + ph - 1] = t1;
     }
     *s_ok = ok;
   } else {
@@ -239,43 +269,116 @@ __device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok) {
 }
 
 // Cout = alpha op(A) B + beta Cin (row-major; op(A) = A (M x K, stride lda) or
-// A^H with A stored K x M; B K x N; Cin may be null when beta = 0).  One
-// 16 x 16 output tile per workgroup at a time, tiles handed out over the
-// grid; the four wavefronts of the workgroup take a quarter of K each and
-// their partial tiles are summed in LDS in a fixed order.  Each wavefront
-// issues the loads of 16 K-steps before their MFMAs, so that one memory
-// latency covers 64 MFMAs (a deeper batch of 24 measured slower).  MFMA f64 16x16x4: lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator item r is D[l / 16 + 4 r][l % 16].
-constexpr int PG_UNROLL = 16;
-template <bool ADJ>
-__device__ void polar_gemm(int64_t M, int64_t N, int64_t K, const double2* __restrict__ A,
-                           int64_t lda, const double2* __restrict__ B, int64_t ldb, double alpha,
-                           double beta, const double2* __restrict__ Cin, double2* __restrict__ C,
-                           int64_t ldc, double* red) {
+// A^H with A stored K x M; B K x N; Cin may be null when beta = 0), by 16 x 16
+// output tiles on FP64 MFMA.  The 8 wavefronts of a workgroup form
+// 8 / WPT groups of WPT wavefronts; each group takes one tile at a time
+// (tiles handed out over the grid), its wavefronts a contiguous share of K
+// each, and the partial tiles are summed in LDS in a fixed order.  The
+// caller picks WPT per product (polar_wpt): a product with few tiles splits K
+// finely, one with many tiles runs several tiles per workgroup at once, so
+// that every product takes about one round of tiles with all of a wavefront's
+// loads in flight together.  Each wavefront issues the loads of up to 8
+// K-steps before their MFMAs.  MFMA f64 16x16x4: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator item r is
+// D[l / 16 + 4 r][l % 16].
+//
+// HERM_C: the product is Hermitian (A^H A with B = A): only the tiles on and
+// above the diagonal (tile row <= tile column) are computed.  HERM_B: B is
+// such a matrix, so its element (k, j) below the diagonal tiles is read as
+// conj(B[j][k]).
+constexpr int PG_UNROLL = 8;
+constexpr int POLAR_THREADS = 512;
+constexpr int POLAR_WAVES = POLAR_THREADS / 64;
+
+// Tile (row, column) of the t-th tile on or above the diagonal of a T x T
+// tile grid, numbered row by row.
+__device__ __forceinline__ void upper_tile(int64_t t, int64_t T, int64_t* tr, int64_t* tc) {
+  int64_t r = 0;
+  while (t >= T - r) {
+    t -= T - r;
+    ++r;
+  }
+  *tr = r;
+  *tc = r + t;
+}
+
+// Wavefronts per tile (8, 4 or 2) for a product of `ntiles` tiles and
+// `ksteps` K-steps of 4 on `grid` workgroups.  The cost of a choice is its
+// rounds of tiles times the time of one round: one memory latency (operands
+// written by other workgroups before the barrier come from HBM, a few us) per
+// batch of PG_UNROLL K-steps, counted as 12 K-steps of MFMA work, plus the
+// K-steps themselves.  Ties go to the finer K split.
+__device__ __forceinline__ int polar_wpt(int ntiles, int ksteps, int grid) {
+  int best = POLAR_WAVES;
+  int best_cost = -1;
+  for (int wpt = POLAR_WAVES; wpt >= 2; wpt >>= 1) {
+    const int groups = POLAR_WAVES / wpt;
+    const int rounds = (ntiles + grid * groups - 1) / (grid * groups);
+    const int per = (ksteps + wpt - 1) / wpt;
+    const int cost = rounds * (12 * ((per + PG_UNROLL - 1) / PG_UNROLL) + per);
+    if (best_cost < 0 || cost < best_cost) {
+      best = wpt;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+template <bool ADJ, bool HERM_C = false, bool HERM_B = false>
+__device__ void polar_gemm(int64_t M64, int64_t N64, int64_t K64, const double2* __restrict__ A,
+                           int64_t lda64, const double2* __restrict__ B, int64_t ldb64,
+                           double alpha, double beta, const double2* __restrict__ Cin,
+                           double2* __restrict__ C, int64_t ldc64, double* red) {
+  // every operand here has fewer than 2^31 elements (checked by the host):
+  // 32-bit offsets from uniform bases keep the address registers few
+  const int M = (int)M64, N = (int)N64, K = (int)K64;
+  const int lda = (int)lda64, ldb = (int)ldb64, ldc = (int)ldc64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t tm = ceil_div(M, 16), tn = ceil_div(N, 16);
+  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
   const int li = lane & 15, lk = lane >> 4;
+  const int ntiles = HERM_C ? tn * (tn + 1) / 2 : tm * tn;
+  const int ksteps = (K + 3) >> 2;
+  const int wpt = polar_wpt(ntiles, ksteps, gridDim.x);
+  const int groups = POLAR_WAVES / wpt;
+  const int grp = wave / wpt, sub = wave % wpt;
   // K range of this wavefront, in whole K-steps of 4
-  const int64_t ksteps = ceil_div(K, 4);
-  const int64_t per = ceil_div(ksteps, 4);
-  const int64_t s0 = wave * per, s1 = (s0 + per < ksteps) ? s0 + per : ksteps;
-  for (int64_t t = blockIdx.x; t < tm * tn; t += gridDim.x) {
-    const int64_t m0 = (t / tn) * 16, n0 = (t % tn) * 16;
-    const int64_t ia = m0 + li, jb = n0 + li;
+  const int per = (ksteps + wpt - 1) / wpt;
+  const int s0 = sub * per, s1 = (s0 + per < ksteps) ? s0 + per : ksteps;
+  // partial tiles: slot (grp, sub - 1) of 64 lanes x 8 doubles
+  double* part = red + (grp * (wpt - 1)) * 64 * 8;
+  for (int base = (int)blockIdx.x * groups; base < ntiles; base += (int)gridDim.x * groups) {
+    const int t = base + grp;
+    const bool active = t < ntiles;
+    int m0 = 0, n0 = 0;
+    if (active) {
+      if (HERM_C) {
+        int64_t tr, tc;
+        upper_tile(t, tn, &tr, &tc);
+        m0 = (int)tr * 16;
+        n0 = (int)tc * 16;
+      } else {
+        m0 = (t / tn) * 16;
+        n0 = (t % tn) * 16;
+      }
+    }
+    const int ia = m0 + li, jb = n0 + li;
     pd4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t sb = s0; sb < s1; sb += PG_UNROLL) {
+    for (int sb = s0; active && sb < s1; sb += PG_UNROLL) {
       double2 av[PG_UNROLL], bv[PG_UNROLL];
 #pragma unroll
       for (int u = 0; u < PG_UNROLL; ++u) {
-        const int64_t k = (sb + u) * 4 + lk;
+        const int k = (sb + u) * 4 + lk;
         double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
         if (sb + u < s1 && k < K) {
-          if (ADJ) {
-            if (ia < M) a = A[k * lda + ia];
-          } else if (ia < M) {
-            a = A[ia * lda + k];
+          if (ia < M) a = ADJ ? A[k * lda + ia] : A[ia * lda + k];
+          if (jb < N) {
+            if (HERM_B && (k >> 4) > (jb >> 4)) {
+              const double2 bt = B[jb * ldb + k];
+              b = make_double2(bt.x, -bt.y);
+            } else {
+              b = B[k * ldb + jb];
+            }
           }
-          if (jb < N) b = B[k * ldb + jb];
         }
         av[u] = a;
         bv[u] = b;
@@ -289,9 +392,10 @@ __device__ void polar_gemm(int64_t M, int64_t N, int64_t K, const double2* __res
         ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bv[u].x, ci, 0, 0, 0);
       }
     }
-    // partial tiles of waves 1..3 to LDS; wave 0 sums them in order and stores
-    if (wave > 0) {
-      double* dst = red + ((wave - 1) * 64 + lane) * 8;
+    // partial tiles of wavefronts 1.. of each group to LDS; wavefront 0 of the
+    // group sums them in order and stores
+    if (active && sub > 0) {
+      double* dst = part + ((sub - 1) * 64 + lane) * 8;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         dst[r] = cr[r];
@@ -299,21 +403,20 @@ __device__ void polar_gemm(int64_t M, int64_t N, int64_t K, const double2* __res
       }
     }
     __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double* src = red + (q * 64 + lane) * 8;
+    if (active && sub == 0) {
+      for (int q = 0; q < wpt - 1; ++q) {
+        const double* src = part + (q * 64 + lane) * 8;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           cr[r] += src[r];
           ci[r] += src[4 + r];
         }
       }
-      const int64_t col = n0 + li;
+      const int col = n0 + li;
       if (col < N) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t row = m0 + lk + 4 * r;
+          const int row = m0 + lk + 4 * r;
           if (row < M) {
             double2 v = make_double2(alpha * cr[r], alpha * ci[r]);
             if (beta != 0.0) {
@@ -330,6 +433,11 @@ __device__ void polar_gemm(int64_t M, int64_t N, int64_t K, const double2* __res
   }
 }
 
+// Newton-Schulz steps that take a smallest singular value s (the others lie
+// in [s, 1]) to within 2.3e-16 of 1: the step maps [0, 1] into itself with
+// fixed point 1, so the whole spectrum is then unitary to a few ulp.  Zero
+// when s already is (then Z = I + X is used as it is: Z^H Z = I - X^2 differs
+// from I by at most |X|^2 <= 4.6e-16).
 __device__ int ns_steps_device(double s) {
   int k = 0;
   while (1.0 - s > 2.3e-16 && k < 60) {
@@ -338,20 +446,20 @@ __device__ int ns_steps_device(double s) {
     if (next == s) break;
     s = next;
   }
-  return k + 1;
+  return k;
 }
 
 // The rotation iteration of the file comment, all of it on the device.
 // XA holds [W; I] on entry and [W V; V] on exit (copied back from the
 // ping-pong buffer if needed); stats[3] is the negligible squared column norm.
-__global__ __launch_bounds__(256) void polar_persistent(int64_t Lg, int64_t l, double2* XA,
+__global__ __launch_bounds__(POLAR_THREADS) void polar_persistent(int64_t Lg, int64_t l, double2* XA,
                                                         double2* XB, double2* G, double2* Z,
                                                         double2* Z2, double2* T,
                                                         const double* __restrict__ stats,
                                                         PolarCtrl* ctrl, double tol,
-                                                        int max_iters) {
-  __shared__ double red_off[4], red_sum[4];
-  __shared__ double red[3 * 64 * 8];
+                                                        int max_iters, long long* trace) {
+  __shared__ double red_off[POLAR_WAVES], red_sum[POLAR_WAVES];
+  __shared__ double red[(POLAR_WAVES - 1) * 64 * 8];
   __shared__ int s_ok;
   unsigned int phase = 0;
   const long long t_start = (long long)wall_clock64();
@@ -360,11 +468,22 @@ __global__ __launch_bounds__(256) void polar_persistent(int64_t Lg, int64_t l, d
   double2* cur = XA;
   double2* nxt = XB;
   int it = 0, converged = 0;
+  long long t_prev = t_start;
+  // workgroup 0 books the time since the previous stamp to prof[slot]
+  auto stamp = [&](int slot) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const long long now = (long long)wall_clock64();
+      ctrl->prof[slot] += now - t_prev;
+      t_prev = now;
+    }
+  };
   for (it = 0; it < max_iters; ++it) {
     // G = W^H W
-    polar_gemm<true>(l, l, Lg, cur, l, cur, l, 1.0, 0.0, nullptr, G, l, red);
-    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
-    // Z = I + X, rows over the workgroups; the statistics of this parity
+    polar_gemm<true, true>(l, l, Lg, cur, l, cur, l, 1.0, 0.0, nullptr, G, l, red);
+    if (!polar_barrier(ctrl, &phase, &s_ok, trace)) break;
+    stamp(4);
+    // Z = I + X, rows over the workgroups (G is read on and above its
+    // diagonal only); the statistics of this parity
     // were cleared one iteration ago
     unsigned long long* st = ctrl->stats[it & 1];
     for (int64_t i = blockIdx.x; i < l; i += gridDim.x) {
@@ -418,7 +537,8 @@ __global__ __launch_bounds__(256) void polar_persistent(int64_t Lg, int64_t l, d
       atomicExch(&ctrl->stats[(it + 1) & 1][0], 0ull);
       atomicExch(&ctrl->stats[(it + 1) & 1][1], 0ull);
     }
-    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
+    if (!polar_barrier(ctrl, &phase, &s_ok, trace)) break;
+    stamp(5);
     const double off = __longlong_as_double(
         (long long)__hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const double xn = __longlong_as_double(
@@ -433,20 +553,24 @@ __global__ __launch_bounds__(256) void polar_persistent(int64_t Lg, int64_t l, d
     // T = f^2 Z^H Z, Z2 = 1.5 f Z - 0.5 f Z T
     const double f = 1.0 / sqrt(1.0 + xn * xn);
     const int ns = ns_steps_device(f);
+    if (ns > POLAR_MAX_NS) break;  // not converged
     bool ok = true;
     for (int k = 0; k < ns && ok; ++k) {
       const double g = (k == 0) ? f : 1.0;
-      polar_gemm<true>(l, l, l, Z, l, Z, l, g * g, 0.0, nullptr, T, l, red);
-      if (!(ok = polar_barrier(ctrl, &phase, &s_ok))) break;
-      polar_gemm<false>(l, l, l, Z, l, T, l, -0.5 * g, 1.5 * g, Z, Z2, l, red);
-      if (!(ok = polar_barrier(ctrl, &phase, &s_ok))) break;
+      polar_gemm<true, true>(l, l, l, Z, l, Z, l, g * g, 0.0, nullptr, T, l, red);
+      if (!(ok = polar_barrier(ctrl, &phase, &s_ok, trace))) break;
+      polar_gemm<false, false, true>(l, l, l, Z, l, T, l, -0.5 * g, 1.5 * g, Z, Z2, l, red);
+      if (!(ok = polar_barrier(ctrl, &phase, &s_ok, trace))) break;
       double2* tmp = Z;
       Z = Z2;
       Z2 = tmp;
     }
     if (!ok) break;
+    stamp(6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->prof[3] += ns;
     polar_gemm<false>(H, l, l, cur, l, Z, l, 1.0, 0.0, nullptr, nxt, l, red);  // [W; V] Z
-    if (!polar_barrier(ctrl, &phase, &s_ok)) break;
+    if (!polar_barrier(ctrl, &phase, &s_ok, trace)) break;
+    stamp(7);
     double2* tmp = cur;
     cur = nxt;
     nxt = tmp;
@@ -477,7 +601,7 @@ __global__ void polar_ctrl_reset(PolarCtrl* c) {
     c->iterations = 0;
     c->converged = 0;
     c->stats[0][0] = c->stats[0][1] = c->stats[1][0] = c->stats[1][1] = 0ull;
-    c->prof[0] = c->prof[1] = c->prof[2] = c->prof[3] = 0;
+    for (int q = 0; q < 8; ++q) c->prof[q] = 0;
   }
 }
 
@@ -505,6 +629,7 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
               double2* VH, const int64_t** tiny_dev, const int** conv_dev, void* work,
               int64_t work_bytes, hipStream_t st_) {
   SEEMPS_CHECK_ARG(Lg >= l && l >= 1, "polar_svd: needs Lg >= l >= 1");
+  SEEMPS_CHECK_ARG((Lg + l) * l < (int64_t)INT32_MAX, "polar_svd: matrix too large");
   PolarWork w = polar_carve(Lg, l, work);
   SEEMPS_CHECK_ARG(work_bytes >= w.bytes, "polar_svd: workspace too small");
   static_assert(sizeof(PolarCtrl) <= 256, "control block too large");
@@ -517,15 +642,13 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
   if (occ_dev != di.device) {
     int o = 0;
     SEEMPS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void*)polar_persistent,
-                                                            256, 0));
+                                                            POLAR_THREADS, 0));
     occ = o;
     occ_dev = di.device;
   }
-  // one workgroup per CU (the widest product has ceil(H/16) ceil(l/16) tiles
-  // of one workgroup each)
+  // one workgroup of 8 wavefronts per CU, no more than the widest product's
+  // ceil(H/16) ceil(l/16) tiles
   int64_t grid = ceil_div(H, 16) * ceil_div(l, 16);
-  // One workgroup per CU: two per CU (where resident) measured slower, the
-  // grid barriers costing more than the extra tiles save.
   if (grid > di.num_cu) grid = di.num_cu;
   if (grid > (int64_t)occ * di.num_cu) grid = (int64_t)occ * di.num_cu;
   if (grid < 1) {
@@ -545,14 +668,21 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
     double tol_ = tol;
     int iters = POLAR_MAX_ITERS;
     const double* stats = w.stats;
+    long long* trace_buf = nullptr;
+    if (trace_level() >= 2) {  // diagnostics: barrier arrival times (leaked per call)
+      const size_t tb = (POLAR_TRACE_BARRIERS * grid + POLAR_TRACE_BARRIERS) * 8;
+      SEEMPS_HIP(hipMalloc(&trace_buf, tb));
+      SEEMPS_HIP(hipMemsetAsync(trace_buf, 0, tb, st_));
+      g_trace_buf = trace_buf;
+    }
     void* args[] = {(void*)&Lg_, (void*)&l_, (void*)&w.XA, (void*)&w.XB, (void*)&w.G,
                     (void*)&w.Z, (void*)&w.Z2, (void*)&w.T, (void*)&stats, (void*)&ctrl,
-                    (void*)&tol_, (void*)&iters};
+                    (void*)&tol_, (void*)&iters, (void*)&trace_buf};
     // nominal SVD flops of the matrix (as for svd_jacobi): 4 (6 Lg l^2 + 16 l^3)
     const double M_ = (double)Lg, N_ = (double)l;
     TimedLaunch timed("polar_svd", st_, 4.0 * (6.0 * M_ * N_ * N_ + 16.0 * N_ * N_ * N_));
     SEEMPS_HIP(hipLaunchCooperativeKernel((const void*)polar_persistent, dim3((unsigned)grid),
-                                          dim3(256), args, 0, st_));
+                                          dim3(POLAR_THREADS), args, 0, st_));
   }
   hipLaunchKernelGGL(polar_norms, dim3((unsigned)l), dim3(256), 0, st_, Lg, l, w.XA, w.nrm2);
   if ((st = check_launch("polar_norms"))) return st;
@@ -564,18 +694,37 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
   hipLaunchKernelGGL(polar_output, dim3(polar_grid(H * l)), dim3(256), 0, st_, Lg, l, w.XA,
                      w.nrm2, w.perm, U, S, VH);
   if ((st = check_launch("polar_output"))) return st;
-  static const bool trace = [] {
-    const char* e = getenv("SEEMPS_POLAR_TRACE");
-    return e && e[0] == '1';
-  }();
-  if (trace) {  // diagnostics: synchronises
+  if (trace_level() >= 1) {  // diagnostics: synchronises
     PolarCtrl h;
     SEEMPS_HIP(hipMemcpyAsync(&h, ctrl, sizeof(h), hipMemcpyDeviceToHost, st_));
     SEEMPS_HIP(hipStreamSynchronize(st_));
     fprintf(stderr, "[polar_svd %lld x %lld] %d iterations, converged %d, %lld barriers, "
             "%.1f us of %.1f us waiting in barriers (workgroup 0), grid %lld (%d per CU "
-            "resident)\n", (long long)Lg, (long long)l, h.iterations, h.converged, h.prof[2],
-            h.prof[0] * 0.01, h.prof[1] * 0.01, (long long)grid, occ);
+            "resident); us in Gram %.1f, generator %.1f, %lld Newton-Schulz steps %.1f, "
+            "rotation %.1f\n", (long long)Lg, (long long)l, h.iterations, h.converged, h.prof[2],
+            h.prof[0] * 0.01, h.prof[1] * 0.01, (long long)grid, occ, h.prof[4] * 0.01,
+            h.prof[5] * 0.01, h.prof[3], h.prof[6] * 0.01, h.prof[7] * 0.01);
+    if (g_trace_buf) {
+      std::vector<long long> tb(POLAR_TRACE_BARRIERS * grid + POLAR_TRACE_BARRIERS);
+      SEEMPS_HIP(hipMemcpy(tb.data(), g_trace_buf, tb.size() * 8, hipMemcpyDeviceToHost));
+      (void)hipFree(g_trace_buf);
+      g_trace_buf = nullptr;
+      const int nb = (int)std::min<long long>(h.prof[2], POLAR_TRACE_BARRIERS);
+      for (int b = 0; b < nb; ++b) {
+        long long lo = tb[b * grid], hi = lo;
+        int last = 0;
+        for (int64_t g = 0; g < grid; ++g) {
+          const long long v = tb[b * grid + g];
+          lo = std::min(lo, v);
+          if (v > hi) { hi = v; last = (int)g; }
+        }
+        const long long dep = tb[POLAR_TRACE_BARRIERS * grid + b];
+        const long long prev = b > 0 ? tb[POLAR_TRACE_BARRIERS * grid + b - 1] : lo;
+        fprintf(stderr, "  barrier %2d: phase %6.1f us (wg0 %6.1f), arrival spread %6.1f us "
+                "(last wg %3d), release %5.1f us\n", b, (hi - prev) * 0.01,
+                (tb[b * grid] - prev) * 0.01, (hi - lo) * 0.01, last, (dep - hi) * 0.01);
+      }
+    }
   }
   if (tiny_dev) *tiny_dev = w.tiny;
   if (conv_dev) *conv_dev = &ctrl->converged;
