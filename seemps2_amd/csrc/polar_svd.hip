@@ -14,7 +14,9 @@
 //
 // The whole iteration runs in one cooperative kernel: the products are 16 x 16
 // output tiles on FP64 MFMA spread over every workgroup, separated by grid
-// barriers, and the convergence measure is read on the device.
+// barriers, and the convergence measure is read on the device.  The Gram
+// product and the generator share one phase: each Gram tile is turned into
+// its tile of Z by the workgroup that computed it.
 //
 // Accuracy: each G_ij is a dot product of two columns, rounded relative to the
 // two column norms, so the rotations are as accurate as one-sided Jacobi's;
@@ -63,7 +65,7 @@ inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 struct PolarWork {
   double2* XA;     // (Lg + l) x l: rows < Lg the working matrix, then V
   double2* XB;     // ping-pong buffer of XA
-  double2* G;      // l x l
+  unsigned long long* rowsum;  // 2 x l: per iteration parity, sums of |X| per row (fixed point)
   double2* Z;      // l x l
   double2* Z2;     // l x l
   double2* T;      // l x l
@@ -87,7 +89,7 @@ PolarWork polar_carve(int64_t Lg, int64_t l, void* base) {
   const int64_t H = Lg + l;
   w.XA = (double2*)take(H * l * 16);
   w.XB = (double2*)take(H * l * 16);
-  w.G = (double2*)take(l * l * 16);
+  w.rowsum = (unsigned long long*)take(2 * l * 8);
   w.Z = (double2*)take(l * l * 16);
   w.Z2 = (double2*)take(l * l * 16);
   w.T = (double2*)take(l * l * 16);
@@ -206,9 +208,9 @@ struct PolarCtrl {
   unsigned int released;  // last barrier phase whose arrivals are complete
   int iterations;
   int converged;
-  unsigned long long stats[2][2];  // per iteration parity: max off, |X| bound (bits)
+  unsigned long long stats[2][2];  // per iteration parity: [0] max off (bits), [1] unused
   // workgroup 0: [0] ticks waiting in barriers, [1] ticks in all, [2] barriers,
-  // [3] Newton-Schulz steps, ticks in [4] Gram, [5] generator, [6] Newton-Schulz,
+  // [3] Newton-Schulz steps, ticks in [4] Gram and generator, [6] Newton-Schulz,
   // [7] rotation phases (each with its closing barrier)
   long long prof[8];
 };
@@ -433,6 +435,161 @@ __device__ void polar_gemm(int64_t M64, int64_t N64, int64_t K64, const double2*
   }
 }
 
+// Fixed-point scale of the per-row sums of |X| (|t| <= 1, so a row sums to at
+// most l < 2^12): integer atomics add them in any order to the same value.
+constexpr double POLAR_ROWSUM_SCALE = 1125899906842624.0;  // 2^50
+
+// The Gram tiles of W (Lg x l) on and above the diagonal, as polar_gemm
+// computes them, fused with the rotation generator: every workgroup also
+// forms the squared norms of the 32 columns of its tile from the operands it
+// loads (the same split of K and summation order in every tile, so a column
+// norm is the same number wherever it is used), and turns its tile of G into
+// the tile of Z = I + X and its mirror below the diagonal.  For a pair (a, b),
+// a < b, the one-sided Jacobi rotation of the pair has tangent t and phase ph
+// (from G_aa, G_bb, G_ab); X_ab = t ph, X_ba = -t conj(ph).  Pairs below the
+// threshold (or with a negligible column) get 0.  Also: st[0] = max over pairs
+// of |G_ab| / sqrt(G_aa G_bb) (atomic max of the bit pattern), rowsum[a] +=
+// sum over the row of |t| in fixed point (integer atomics).
+__device__ void polar_gram_generator(int Lg, int l, const double2* __restrict__ W,
+                                     double2* __restrict__ Z, double tiny2, double tol,
+                                     unsigned long long* st, unsigned long long* rowsum,
+                                     double* red) {
+  constexpr int SLOT = 10;  // partial tile (8) and the two column norms
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tn = (l + 15) >> 4;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ntiles = tn * (tn + 1) / 2;
+  const int ksteps = (Lg + 3) >> 2;
+  const int wpt = polar_wpt(ntiles, ksteps, gridDim.x);
+  const int groups = POLAR_WAVES / wpt;
+  const int grp = wave / wpt, sub = wave % wpt;
+  const int per = (ksteps + wpt - 1) / wpt;
+  const int s0 = sub * per, s1 = (s0 + per < ksteps) ? s0 + per : ksteps;
+  double* part = red + (grp * (wpt - 1)) * 64 * SLOT;
+  for (int base = (int)blockIdx.x * groups; base < ntiles; base += (int)gridDim.x * groups) {
+    const int t = base + grp;
+    const bool active = t < ntiles;
+    int m0 = 0, n0 = 0;
+    if (active) {
+      int64_t tr, tc;
+      upper_tile(t, tn, &tr, &tc);
+      m0 = (int)tr * 16;
+      n0 = (int)tc * 16;
+    }
+    const int ia = m0 + li, jb = n0 + li;
+    pd4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    double na = 0.0, nb = 0.0;
+    for (int sb = s0; active && sb < s1; sb += PG_UNROLL) {
+      double2 av[PG_UNROLL], bv[PG_UNROLL];
+#pragma unroll
+      for (int u = 0; u < PG_UNROLL; ++u) {
+        const int k = (sb + u) * 4 + lk;
+        double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+        if (sb + u < s1 && k < Lg) {
+          if (ia < l) a = W[k * l + ia];
+          if (jb < l) b = W[k * l + jb];
+        }
+        av[u] = a;
+        bv[u] = b;
+      }
+#pragma unroll
+      for (int u = 0; u < PG_UNROLL; ++u) {
+        const double ar = av[u].x, ai = -av[u].y;
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bv[u].x, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bv[u].y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bv[u].y, ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bv[u].x, ci, 0, 0, 0);
+        na = fma(av[u].x, av[u].x, fma(av[u].y, av[u].y, na));
+        nb = fma(bv[u].x, bv[u].x, fma(bv[u].y, bv[u].y, nb));
+      }
+    }
+    // column norms: the four lanes of a column (lk = 0..3), summed pairwise
+    // (the same value lands in all four)
+    na += __shfl_xor(na, 16, 64);
+    na += __shfl_xor(na, 32, 64);
+    nb += __shfl_xor(nb, 16, 64);
+    nb += __shfl_xor(nb, 32, 64);
+    if (active && sub > 0) {
+      double* dst = part + ((sub - 1) * 64 + lane) * SLOT;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        dst[r] = cr[r];
+        dst[4 + r] = ci[r];
+      }
+      dst[8] = na;
+      dst[9] = nb;
+    }
+    __syncthreads();
+    if (active && sub == 0) {
+      for (int q = 0; q < wpt - 1; ++q) {
+        const double* src = part + (q * 64 + lane) * SLOT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          cr[r] += src[r];
+          ci[r] += src[4 + r];
+        }
+        na += src[8];
+        nb += src[9];
+      }
+      // tile entry r of this lane: G_ab, a = m0 + lk + 4 r, b = n0 + li;
+      // G_aa from the lane that loaded column a (li = lk + 4 r, lk = 0)
+      const int b = jb;
+      const double gbb = nb;
+      double mx = 0.0, sum_b = 0.0;
+      double sum_a[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = m0 + lk + 4 * r;
+        const double gaa = __shfl(na, lk + 4 * r, 64);
+        sum_a[r] = 0.0;
+        if (a >= l || b >= l || a > b) continue;
+        if (a == b) {
+          Z[a * l + a] = make_double2(1.0, 0.0);
+          continue;
+        }
+        double2 z = make_double2(0.0, 0.0);
+        const double2 c = make_double2(cr[r], ci[r]);
+        const double s2 = cabs2(c);
+        if (gaa > tiny2 && gbb > tiny2) mx = fmax(mx, sqrt(s2 / gaa / gbb));
+        if (s2 > 0.0 && gaa > tiny2 && gbb > tiny2 && s2 > (tol * tol * gaa) * gbb) {
+          const double rr = jac::rsqrt_nr(s2);
+          const double2 ph = make_double2(c.x * rr, c.y * rr);
+          const double tau = (gbb - gaa) * (0.5 * rr);
+          double tt;
+          if (fabs(tau) > 1e150) {
+            tt = 0.5 * jac::rcp_nr(tau);
+          } else {
+            const double q = fma(tau, tau, 1.0);
+            tt = (tau >= 0.0 ? 1.0 : -1.0) * jac::rcp_nr(fabs(tau) + q * jac::rsqrt_nr(q));
+          }
+          z = make_double2(tt * ph.x, tt * ph.y);
+          sum_a[r] = fabs(tt);
+          sum_b += fabs(tt);
+        }
+        Z[a * l + b] = z;
+        Z[b * l + a] = make_double2(-z.x, z.y);
+      }
+      // row b: the four lanes of column b (lk = 0..3); rows a: the sixteen
+      // lanes of each (lk, r)
+      sum_b += __shfl_xor(sum_b, 16, 64);
+      sum_b += __shfl_xor(sum_b, 32, 64);
+      if (lk == 0 && b < l && sum_b > 0.0)
+        atomicAdd(&rowsum[b], (unsigned long long)ceil(sum_b * POLAR_ROWSUM_SCALE));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = sum_a[r];
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+        const int a = m0 + lk + 4 * r;
+        if (li == 0 && a < l && v > 0.0)
+          atomicAdd(&rowsum[a], (unsigned long long)ceil(v * POLAR_ROWSUM_SCALE));
+      }
+      for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+      if (lane == 0) atomicMax(&st[0], (unsigned long long)__double_as_longlong(mx));
+    }
+    __syncthreads();
+  }
+}
+
 // Newton-Schulz steps that take a smallest singular value s (the others lie
 // in [s, 1]) to within 2.3e-16 of 1: the step maps [0, 1] into itself with
 // fixed point 1, so the whole spectrum is then unitary to a few ulp.  Zero
@@ -453,13 +610,14 @@ __device__ int ns_steps_device(double s) {
 // XA holds [W; I] on entry and [W V; V] on exit (copied back from the
 // ping-pong buffer if needed); stats[3] is the negligible squared column norm.
 __global__ __launch_bounds__(POLAR_THREADS) void polar_persistent(int64_t Lg, int64_t l, double2* XA,
-                                                        double2* XB, double2* G, double2* Z,
+                                                        double2* XB,
+                                                        unsigned long long* rowsum, double2* Z,
                                                         double2* Z2, double2* T,
                                                         const double* __restrict__ stats,
                                                         PolarCtrl* ctrl, double tol,
                                                         int max_iters, long long* trace) {
-  __shared__ double red_off[POLAR_WAVES], red_sum[POLAR_WAVES];
-  __shared__ double red[(POLAR_WAVES - 1) * 64 * 8];
+  __shared__ unsigned long long red_max[POLAR_WAVES];
+  __shared__ double red[(POLAR_WAVES - 1) * 64 * 10];
   __shared__ int s_ok;
   unsigned int phase = 0;
   const long long t_start = (long long)wall_clock64();
@@ -478,71 +636,34 @@ __global__ __launch_bounds__(POLAR_THREADS) void polar_persistent(int64_t Lg, in
     }
   };
   for (it = 0; it < max_iters; ++it) {
-    // G = W^H W
-    polar_gemm<true, true>(l, l, Lg, cur, l, cur, l, 1.0, 0.0, nullptr, G, l, red);
+    // Gram tiles fused with the generator: Z = I + X, the largest normalised
+    // off-diagonal in st[0] and the row sums of |X| in rowsum[it & 1] (both
+    // cleared one iteration ago, or before the launch)
+    unsigned long long* st = ctrl->stats[it & 1];
+    unsigned long long* rs = rowsum + (it & 1) * l;
+    polar_gram_generator((int)Lg, (int)l, cur, Z, tiny2, tol, st, rs, red);
+    if (blockIdx.x == 0) {
+      unsigned long long* rs_next = rowsum + ((it + 1) & 1) * l;
+      for (int64_t a = threadIdx.x; a < l; a += blockDim.x) rs_next[a] = 0ull;
+      if (threadIdx.x == 0) atomicExch(&ctrl->stats[(it + 1) & 1][0], 0ull);
+    }
     if (!polar_barrier(ctrl, &phase, &s_ok, trace)) break;
     stamp(4);
-    // Z = I + X, rows over the workgroups (G is read on and above its
-    // diagonal only); the statistics of this parity
-    // were cleared one iteration ago
-    unsigned long long* st = ctrl->stats[it & 1];
-    for (int64_t i = blockIdx.x; i < l; i += gridDim.x) {
-      double mx = 0.0, rowsum = 0.0;
-      for (int64_t j = threadIdx.x; j < l; j += blockDim.x) {
-        double2 z = make_double2(i == j ? 1.0 : 0.0, 0.0);
-        if (j != i) {
-          const int64_t a = i < j ? i : j, b = i < j ? j : i;
-          const double gaa = G[a * l + a].x, gbb = G[b * l + b].x;
-          const double2 c = G[a * l + b];
-          const double s2 = cabs2(c);
-          if (gaa > tiny2 && gbb > tiny2) mx = fmax(mx, sqrt(s2 / gaa / gbb));
-          if (s2 > 0.0 && gaa > tiny2 && gbb > tiny2 && s2 > (tol * tol * gaa) * gbb) {
-            const double r = jac::rsqrt_nr(s2);
-            const double2 ph = make_double2(c.x * r, c.y * r);
-            const double tau = (gbb - gaa) * (0.5 * r);
-            double t;
-            if (fabs(tau) > 1e150) {
-              t = 0.5 * jac::rcp_nr(tau);
-            } else {
-              const double q = fma(tau, tau, 1.0);
-              t = (tau >= 0.0 ? 1.0 : -1.0) * jac::rcp_nr(fabs(tau) + q * jac::rsqrt_nr(q));
-            }
-            z = (i == a) ? make_double2(t * ph.x, t * ph.y) : make_double2(-t * ph.x, t * ph.y);
-            rowsum += fabs(t);
-          }
-        }
-        Z[i * l + j] = z;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        mx = fmax(mx, __shfl_down(mx, off, 64));
-        rowsum += __shfl_down(rowsum, off, 64);
-      }
-      if ((threadIdx.x & 63) == 0) {
-        red_off[threadIdx.x >> 6] = mx;
-        red_sum[threadIdx.x >> 6] = rowsum;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double m = 0.0, su = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
-          m = fmax(m, red_off[q]);
-          su += red_sum[q];
-        }
-        atomicMax(&st[0], (unsigned long long)__double_as_longlong(m));
-        atomicMax(&st[1], (unsigned long long)__double_as_longlong(su));
-      }
-      __syncthreads();
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      atomicExch(&ctrl->stats[(it + 1) & 1][0], 0ull);
-      atomicExch(&ctrl->stats[(it + 1) & 1][1], 0ull);
-    }
-    if (!polar_barrier(ctrl, &phase, &s_ok, trace)) break;
-    stamp(5);
     const double off = __longlong_as_double(
         (long long)__hip_atomic_load(&st[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const double xn = __longlong_as_double(
-        (long long)__hip_atomic_load(&st[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    // ||X||_2 <= max row sum of |X| (X anti-Hermitian: the 1- and inf-norms agree)
+    unsigned long long mrow = 0ull;
+    for (int64_t a = threadIdx.x; a < l; a += blockDim.x) mrow = rs[a] > mrow ? rs[a] : mrow;
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long v = __shfl_xor(mrow, o, 64);
+      mrow = v > mrow ? v : mrow;
+    }
+    if ((threadIdx.x & 63) == 0) red_max[threadIdx.x >> 6] = mrow;
+    __syncthreads();
+    unsigned long long mall = 0ull;
+    for (int q = 0; q < POLAR_WAVES; ++q) mall = red_max[q] > mall ? red_max[q] : mall;
+    __syncthreads();
+    const double xn = (double)mall / POLAR_ROWSUM_SCALE;
     if (!(off == off)) break;  // NaN: not converged
     if (off < tol || xn == 0.0) {
       converged = 1;
@@ -593,7 +714,9 @@ __global__ __launch_bounds__(POLAR_THREADS) void polar_persistent(int64_t Lg, in
   }
 }
 
-__global__ void polar_ctrl_reset(PolarCtrl* c) {
+// Control block and both parities of the row sums to zero.
+__global__ void polar_ctrl_reset(PolarCtrl* c, int64_t l, unsigned long long* rowsum) {
+  for (int64_t a = threadIdx.x; a < 2 * l; a += blockDim.x) rowsum[a] = 0ull;
   if (threadIdx.x == 0) {
     c->barrier = 0;
     c->abort = 0;
@@ -661,7 +784,7 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
   hipLaunchKernelGGL(polar_frob, dim3(1), dim3(1024), 0, st_, Lg, l, w.XA, w.stats);
   if ((st = check_launch("polar_frob"))) return st;
   PolarCtrl* ctrl = (PolarCtrl*)w.ctrl;
-  hipLaunchKernelGGL(polar_ctrl_reset, dim3(1), dim3(64), 0, st_, ctrl);
+  hipLaunchKernelGGL(polar_ctrl_reset, dim3(1), dim3(256), 0, st_, ctrl, l, w.rowsum);
   if ((st = check_launch("polar_ctrl_reset"))) return st;
   {
     int64_t Lg_ = Lg, l_ = l;
@@ -675,7 +798,7 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
       SEEMPS_HIP(hipMemsetAsync(trace_buf, 0, tb, st_));
       g_trace_buf = trace_buf;
     }
-    void* args[] = {(void*)&Lg_, (void*)&l_, (void*)&w.XA, (void*)&w.XB, (void*)&w.G,
+    void* args[] = {(void*)&Lg_, (void*)&l_, (void*)&w.XA, (void*)&w.XB, (void*)&w.rowsum,
                     (void*)&w.Z, (void*)&w.Z2, (void*)&w.T, (void*)&stats, (void*)&ctrl,
                     (void*)&tol_, (void*)&iters, (void*)&trace_buf};
     // nominal SVD flops of the matrix (as for svd_jacobi): 4 (6 Lg l^2 + 16 l^3)
@@ -700,10 +823,10 @@ int polar_svd(int64_t Lg, int64_t l, const double2* W, int64_t ldw, double2* U, 
     SEEMPS_HIP(hipStreamSynchronize(st_));
     fprintf(stderr, "[polar_svd %lld x %lld] %d iterations, converged %d, %lld barriers, "
             "%.1f us of %.1f us waiting in barriers (workgroup 0), grid %lld (%d per CU "
-            "resident); us in Gram %.1f, generator %.1f, %lld Newton-Schulz steps %.1f, "
+            "resident); us in Gram and generator %.1f, %lld Newton-Schulz steps %.1f, "
             "rotation %.1f\n", (long long)Lg, (long long)l, h.iterations, h.converged, h.prof[2],
             h.prof[0] * 0.01, h.prof[1] * 0.01, (long long)grid, occ, h.prof[4] * 0.01,
-            h.prof[5] * 0.01, h.prof[3], h.prof[6] * 0.01, h.prof[7] * 0.01);
+            h.prof[3], h.prof[6] * 0.01, h.prof[7] * 0.01);
     if (g_trace_buf) {
       std::vector<long long> tb(POLAR_TRACE_BARRIERS * grid + POLAR_TRACE_BARRIERS);
       SEEMPS_HIP(hipMemcpy(tb.data(), g_trace_buf, tb.size() * 8, hipMemcpyDeviceToHost));
