@@ -171,10 +171,15 @@ class LearnedTruncation:
     basis_splits: int = 0
     sketch_columns: int = 0
 
-    def sketch_width(self, feats: torch.Tensor, m: int, n: int, max_bond: int) -> int:
-        r = min(m, n)
+    def predicted_fraction(self, feats: torch.Tensor) -> float:
         frac = float(self.predictor.predict(feats.reshape(1, -1))[0].item())
-        frac = min(max(frac, 0.0), 1.0)
+        return min(max(frac, 0.0), 1.0)
+
+    def sketch_width(self, feats: torch.Tensor, m: int, n: int, max_bond: int,
+                     frac: float | None = None) -> int:
+        r = min(m, n)
+        if frac is None:
+            frac = self.predicted_fraction(feats)
         l = int(math.ceil(self.margin * frac * r)) + self.oversample
         return min(l, max_bond + self.oversample, r)
 
@@ -199,13 +204,17 @@ class LearnedTruncation:
                 # no width needs predicting
                 return self._basis_split(AA2, m, n, mode, basis, k, 0, strategy)
         feats = spectral_features(AA2, m, n, seed=self.seed + self.calls)
-        l = self.sketch_width(feats, m, n, strategy.max_bond_dimension)
-        if k is not None:
+        frac = self.predicted_fraction(feats)
+        l = self.sketch_width(feats, m, n, strategy.max_bond_dimension, frac)
+        need = min(int(math.ceil(frac * r)), strategy.max_bond_dimension)
+        if k is not None and need - k <= MAX_EXPANSION:
             # at most MAX_EXPANSION new directions per split: the expansion
             # block is then rotated to orthogonal images in one workgroup and
-            # the sketched SVD stays the fast rotation iteration; a bond that
-            # needs to grow further is caught by the truncation (the weight
-            # outside the sketch exceeds the budget) and split in full
+            # the sketched SVD stays the fast rotation iteration.  A bond
+            # predicted to grow by more skips the basis (its split would fall
+            # back); one that grows more than predicted is caught by the
+            # truncation (the weight outside the sketch exceeds the budget)
+            # and split in full
             p = min(max(self.oversample, l - k), MAX_EXPANSION)
             if k + p < r:
                 return self._basis_split(AA2, m, n, mode, basis, k, p, strategy)
