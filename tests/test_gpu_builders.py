@@ -74,7 +74,7 @@ def test_function_loading_laplacian_euler(S):
     d1 = D1.to_matrix()
     ksum = (np.kron(d1, np.eye(2 ** n)) + np.kron(np.eye(2 ** n), d1)).reshape([2] * (4 * n))
     axes = list(perm) + [2 * n + p for p in perm]
-    assert np.abs(dense - ksum.transpose(axes).reshape(4 ** n, 4 ** n)).max() < 1e-9 * np.abs(dense).max()
+    assert np.abs(dense - ksum.transpose(axes).reshape(4 ** n, 4 ** n)).max() < 1e-13 * np.abs(dense).max()
     f = np.kron(fx, fy).reshape([2] * (2 * n)).transpose(perm).reshape(-1)
     assert np.abs(psi.to_vector() - f).max() < 1e-12
     dt, steps = 1e-6, 5

@@ -237,11 +237,11 @@ def test_two_site_matvec_and_eigensolve(K, rng):
         ek = np.zeros(n, complex)
         ek[k] = 1
         Hd[:, k] = oal.two_site_matvec(L, H12, R, ek.reshape(v.shape)).reshape(-1)
-    assert np.abs(Hd - Hd.conj().T).max() < 1e-10 * np.abs(Hd).max()
+    assert np.abs(Hd - Hd.conj().T).max() < 1e-14 * np.abs(Hd).max()
     w, V = np.linalg.eigh(Hd)
     lam, vec, stats = K.eigensolve(K.as_dev(L), Hm, K.as_dev(R), K.as_dev(v), d, d, ncv=20,
                                    tol=1e-12)
-    assert abs(lam - w[0]) < 1e-9 * max(1, abs(w[0]))
+    assert abs(lam - w[0]) < 1e-12 * max(1, abs(w[0]))
     x = K.to_numpy(vec).reshape(-1)
     assert abs(abs(np.vdot(V[:, 0], x)) - 1) < 1e-8
     # mixed precision: from a random start the first cycle runs FP32 products
