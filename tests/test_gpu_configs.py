@@ -409,42 +409,77 @@ def test_config5_haar_layers_learned_vs_full_n48_d4_chi512(S):
 
 
 def test_config4_many_uneven_sums(S):
-    """Random MPS pairs of uneven length (N ~ U[10, 40]) and bond dimension
-    (chi ~ U[8, 128]), half of them complex64, each sum simplified to
-    tolerance 1e-8.  Every result against the exact sum (fidelity from
-    overlaps), two of them against the oracle on the same inputs (bond
-    dimensions and states)."""
+    """The workload at its stated size: 512 random MPS pairs of uneven length
+    (N ~ U[10, 40]) and bond dimensions (chi ~ U[8, 128] for each state), half
+    of them complex64, each sum psi_k + phi_k simplified to tolerance 1e-8.
+    Every result against its exact sum, by fidelity from the 6 x 512 overlaps
+    <out|psi>, <out|phi>, <out|out>, <psi|psi>, <phi|phi>, <psi|phi> of one
+    batched (graph-captured) overlap launch; the first two pairs (small
+    enough for the oracle's NumPy SVDs) also against the oracle on the same
+    inputs (bond dimensions and states)."""
+    import time
+
+    from seemps2_amd.batch import BatchedOverlap
+
     rng = np.random.default_rng(404)
+    gen = torch.Generator(device="cuda").manual_seed(404)
     st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
     ost = oc.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
-    worst, checked = 0.0, 0  # worst infidelity relative to its bound
-    for k in range(12):
-        # the first two are small enough for the oracle's NumPy SVDs and a
-        # dense comparison
-        N = int(rng.integers(10, 41)) if k >= 2 else int(rng.integers(10, 17))
-        chi = int(rng.integers(8, 129)) if k >= 2 else int(rng.integers(8, 33))
-        a = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
-        b = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
-        if k % 2:
-            a = oa.OMPS([A.astype(np.complex64).astype(np.complex128) for A in a.tensors], 0.0)
-            da = S.MPS([A.astype(np.complex64) for A in a.tensors])
+
+    def device_state(N, chi, dtype):
+        dims = [min(chi, 2 ** min(i, N - i)) for i in range(N + 1)]
+        return S.MPS([(torch.randn(dims[i], 2, dims[i + 1], dtype=torch.complex128,
+                                   device="cuda", generator=gen) / np.sqrt(2 * dims[i]))
+                      .to(dtype).to(torch.complex128) for i in range(N)])
+
+    pairs, outs, Ns, checked = [], [], [], 0
+    t_simplify = 0.0
+    for k in range(512):
+        if k < 2:
+            N = int(rng.integers(10, 17))
+            chi = int(rng.integers(8, 33))
+            a = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
+            b = ob.random_mps([2] * N, D=chi, complex=True, rng=rng)
+            if k % 2:
+                a = oa.OMPS([A.astype(np.complex64).astype(np.complex128) for A in a.tensors], 0.0)
+                da = S.MPS([A.astype(np.complex64) for A in a.tensors])
+            else:
+                da = S.MPS(a.tensors)
+            db = S.MPS(b.tensors)
         else:
-            da = S.MPS(a.tensors)
-        db = S.MPS(b.tensors)
+            N = int(rng.integers(10, 41))
+            dt = torch.complex64 if k % 2 else torch.complex128
+            da = device_state(N, int(rng.integers(8, 129)), dt)
+            db = device_state(N, int(rng.integers(8, 129)), dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         out = S.simplify(S.MPSSum([1.0, 1.0], [da, db]), strategy=st)
-        ov = complex(S.scprod(out, da)) + complex(S.scprod(out, db))
-        n_out = abs(complex(S.scprod(out, out)))
-        n_sum = abs(complex(S.scprod(da, da)) + complex(S.scprod(db, db))
-                    + 2 * complex(S.scprod(da, db)).real)
-        fid = abs(ov) ** 2 / (n_out * n_sum)
-        # each of the N - 1 truncations may drop up to 1e-8 of the norm squared
-        worst = max(worst, (1 - fid) / ((N - 1) * 1e-8))
+        torch.cuda.synchronize()
+        t_simplify += time.perf_counter() - t0
         if k < 2:
             ref = oa.simplify(oa.OMPSSum([1.0, 1.0], [a, b]), ost)
             assert out.bond_dimensions() == ref.bond_dimensions()
             assert rel(out.to_vector(), ref.to_vector()) < 1e-12
             checked += 1
-        report("config4 state", k=k, N=N, chi=chi, max_bond=out.max_bond_dimension(),
-               fidelity=fid)
-    report("config4", states=12, worst_infidelity_over_bound=worst, oracle_checked=checked)
-    assert worst <= 1.0
+        pairs.append((da, db))
+        outs.append(out)
+        Ns.append(N)
+    bras = [o for o in outs] * 3 + [a for a, _ in pairs] * 2 + [b for _, b in pairs]
+    kets = ([a for a, _ in pairs] + [b for _, b in pairs] + outs + [a for a, _ in pairs]
+            + [b for _, b in pairs] + [b for _, b in pairs])
+    batch = BatchedOverlap(bras, kets, graph=True)
+    ov = batch.run()
+    again = batch.run()  # a second replay of the captured graph gives the same
+    assert np.array_equal(ov, again)
+    o_a, o_b, o_o, a_a, a_b, b_b = ov.reshape(6, 512)
+    fid = np.abs(o_a + o_b) ** 2 / (np.abs(o_o) * np.abs(a_a + b_b + 2 * a_b.real))
+    # each of the N - 1 truncations may drop up to 1e-8 of the norm squared
+    ratio = (1 - fid) / ((np.array(Ns) - 1) * 1e-8)
+    # the batched overlaps against the per-pair ones on a few states
+    for k in (0, 1, 300, 511):
+        assert abs(o_a[k] - complex(S.scprod(outs[k], pairs[k][0]))) <= 1e-12 * abs(o_a[k])
+    report("config4", states=512, states_per_s=512 / t_simplify, worst_fidelity=fid.min(),
+           worst_infidelity_over_bound=ratio.max(),
+           max_bond=max(o.max_bond_dimension() for o in outs), oracle_checked=checked)
+    assert checked == 2
+    assert ratio.max() <= 1.0
