@@ -165,11 +165,32 @@ class LearnedTruncation:
     min_size: int = 64
     seed: int = 1234
     use_basis: bool = True
+    #: after this many sketched splits in a row fell back to the full SVD (a
+    #: spectrum the sketches cannot certify, such as the flat spectra of
+    #: random circuits), the next ``backoff`` splits go to the full SVD
+    #: directly; the pause doubles after every streak, up to 64 times itself
+    fallback_streak: int = 8
+    backoff: int = 8
     calls: int = 0
     fallbacks: int = 0
     full_splits: int = 0
     basis_splits: int = 0
     sketch_columns: int = 0
+    _streak: int = 0
+    _skip: int = 0
+    _pause: int = 0
+
+    def _record(self, fb: bool) -> None:
+        self.fallbacks += int(fb)
+        if not fb:
+            self._streak = 0
+            self._pause = 0
+            return
+        self._streak += 1
+        if self._streak >= self.fallback_streak:
+            self._pause = min(max(self.backoff, 2 * self._pause), 64 * self.backoff)
+            self._skip = self._pause
+            self._streak = 0
 
     def predicted_fraction(self, feats: torch.Tensor) -> float:
         frac = float(self.predictor.predict(feats.reshape(1, -1))[0].item())
@@ -191,7 +212,9 @@ class LearnedTruncation:
         a x d x k, left-canonical), or None."""
         r = min(m, n)
         self.calls += 1
-        if r < self.min_size:
+        if r < self.min_size or self._skip > 0:
+            if r >= self.min_size:
+                self._skip -= 1
             self.full_splits += 1
             return K.svd_split(AA2, m, n, mode, strategy.method, strategy.tolerance,
                                strategy.max_bond_dimension)[:4]
@@ -228,7 +251,7 @@ class LearnedTruncation:
                                                   power_iters=self.power_iters,
                                                   seed=self.seed + 7919 * self.calls)
         self.sketch_columns += l
-        self.fallbacks += int(fb)
+        self._record(fb)
         return A, B, D, err2
 
     def _basis_split(self, AA2, m, n, mode, basis, k, p, strategy):
@@ -238,7 +261,7 @@ class LearnedTruncation:
             strategy.max_bond_dimension, seed=self.seed + 7919 * self.calls)
         self.basis_splits += 1
         self.sketch_columns += k + p
-        self.fallbacks += int(fb)
+        self._record(fb)
         return A, B, D, err2
 
 

@@ -113,3 +113,58 @@ def test_strategy_api(S):
     with pytest.raises(AssertionError):
         S.Strategy(tolerance=2.0)
     assert "RelativeNorm" in str(S.DEFAULT_STRATEGY)
+
+
+def test_learned_truncation_backs_off_after_fallbacks(monkeypatch):
+    """Sketched splits that keep falling back pause the sketching: after
+    `fallback_streak` fallbacks in a row the next `backoff` eligible splits go
+    to the full SVD, the pause doubling with every further streak and
+    resetting after a sketched split that holds."""
+    import torch
+    from seemps2_amd import learned
+
+    calls = []
+
+    def full(AA, m, n, mode, *a):
+        calls.append("full")
+        return None, None, 1, 0.0, None
+
+    outcome = {"fb": True}
+
+    def sketch(AA, m, n, mode, *a, **k):
+        calls.append("sketch")
+        return None, None, 1, 0.0, None, outcome["fb"]
+
+    class Half:
+        def predict(self, X):
+            return torch.full((X.shape[0],), 0.5)
+
+    monkeypatch.setattr(learned.K, "svd_split", full)
+    monkeypatch.setattr(learned.K, "svd_split_sketch", sketch)
+    monkeypatch.setattr(learned, "spectral_features", lambda *a, **k: torch.zeros(32))
+    lt = learned.LearnedTruncation(Half(), min_size=8, fallback_streak=2, backoff=3)
+    strategy = type("St", (), {"method": 2, "tolerance": 1e-8, "max_bond_dimension": 100})()
+    AA = torch.zeros(1)
+
+    def run(k):
+        for _ in range(k):
+            lt.split(AA, 100, 100, 0, strategy)
+
+    run(2)                       # two fallbacks: streak reached
+    assert calls == ["sketch", "sketch"]
+    run(3)                       # paused for 3
+    assert calls[2:] == ["full"] * 3
+    run(2)                       # two more fallbacks: pause doubles to 6
+    run(6)
+    assert calls[5:] == ["sketch", "sketch"] + ["full"] * 6
+    outcome["fb"] = False
+    run(1)                       # a sketch that holds resets the pause
+    outcome["fb"] = True
+    run(2)
+    run(3)
+    assert calls[13:] == ["sketch", "sketch", "sketch"] + ["full"] * 3
+    assert lt.fallbacks == 6 and lt.full_splits == 12
+    # splits below min_size do not consume the pause
+    lt._skip = 2
+    lt.split(AA, 4, 4, 0, strategy)
+    assert lt._skip == 2
