@@ -828,14 +828,15 @@ extern "C" int seemps_svd_split_basis(int64_t m, int64_t n, const void* A_, int 
     }
   }
   int off = 0;
-  // W = X [basis | expansion].  When the sketch is the basis alone (p = 0: a
-  // bond at its cap), W is the new two-site matrix seen through the previous
-  // singular vectors and its columns are orthogonal up to the change of the
-  // two-site tensor: its SVD is the simultaneous rotation iteration of
-  // polar_svd.hip, a few GEMM iterations.  Expansion columns are not
-  // orthogonal to the rest, and the block Jacobi (which converges from any
-  // start) takes those matrices, and any the rotation iteration does not
-  // finish.  SEEMPS_BASIS_SVD=jacobi uses the block Jacobi throughout.
+  // W = X [basis | expansion].  The basis columns are the new two-site matrix
+  // seen through the previous singular vectors, orthogonal up to the change of
+  // the two-site tensor, and the expansion block was rotated above so that its
+  // images are orthogonal among themselves (and nearly so to the rest): the
+  // SVD of W is the simultaneous rotation iteration of polar_svd.hip, a few
+  // GEMM iterations, for up to jac::PW expansion columns.  The block Jacobi
+  // (which converges from any start) takes wider sketches and any matrix the
+  // rotation iteration does not finish.  SEEMPS_BASIS_SVD=jacobi uses the
+  // block Jacobi throughout.
   static const bool polar_enabled = [] {
     const char* e = getenv("SEEMPS_BASIS_SVD");
     return !(e && strcmp(e, "jacobi") == 0);
