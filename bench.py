@@ -140,7 +140,72 @@ def rocprof_summary(directory, bench_json=None, top=25):
                         out.append(f"HIP-event average less the event gaps ({br} - {e_us:.2f} "
                                    f"us): {ev - (br - e_us):.2f} us against the trace's "
                                    f"{avg:.1f} us.")
+    if trace:
+        out += chosen_form_breakdown(trace[0])
     return "\n".join(out)
+
+
+def chosen_form_breakdown(trace_csv, top=15):
+    """Kernel totals between the first and the second block of empty
+    calibration kernels: the chosen form's training and timed sweeps (the
+    plain form's timed sweeps end with the first block, the chosen form's
+    with the second)."""
+    import csv
+
+    rows = []
+    for r in csv.DictReader(open(trace_csv)):
+        rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                     kernel_base_name(r.get("Kernel_Name", ""))))
+    rows.sort()
+    marks = [i for i, (_, _, k) in enumerate(rows) if k == "timing_empty_kernel"]
+    if not marks:
+        return []
+    # calibration blocks: runs of consecutive marker dispatches
+    blocks, start = [], marks[0]
+    for a, b in zip(marks, marks[1:] + [None]):
+        if b is None or b != a + 1:
+            blocks.append((start, a))
+            start = b
+    if len(blocks) < 2:
+        return []
+    seg = rows[blocks[0][1] + 1:blocks[1][0]]
+    if not seg:
+        return []
+    span = (seg[-1][1] - seg[0][0]) / 1e6
+    acc = {}
+    for t0, t1, k in seg:
+        c, d = acc.get(k, (0, 0))
+        acc[k] = (c + 1, d + t1 - t0)
+    busy = sum(d for _, d in acc.values()) / 1e6
+    # the timed sweeps follow the predictor's training (its gradient and Adam
+    # kernels; the forward kernel also runs inside the sweeps)
+    last_train = max((i for i, (_, _, k) in enumerate(seg)
+                      if k in ("dense_weight_grad", "adam_kernel")), default=-1)
+    sweeps = seg[last_train + 1:]
+    s_wall = (sweeps[-1][1] - sweeps[0][0]) / 1e6 if sweeps else 0.0
+    s_busy = sum(t1 - t0 for t0, t1, _ in sweeps) / 1e6
+    out = ["", f"Chosen form (training and timed sweeps, between the two calibration blocks): "
+               f"{span:.1f} ms wall, kernels busy {busy:.1f} ms; its timed sweeps alone "
+               f"(after the training kernels): {s_wall:.1f} ms wall, kernels busy {s_busy:.1f} ms "
+               f"({100 * s_busy / max(s_wall, 1e-9):.0f} %), {len(sweeps)} dispatches.", "",
+           "| kernel | calls | total ms | avg us | % of busy |", "|---|---|---|---|---|"]
+    for k, (c, d) in sorted(acc.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| `{k}` | {c} | {d / 1e6:.2f} | {d / c / 1e3:.1f} | {100 * d / 1e6 / busy:.1f} |")
+    # idle gaps inside the timed sweeps, by the kernels on either side
+    gaps = {}
+    for (a0, a1, ka), (b0, b1, kb) in zip(sweeps, sweeps[1:]):
+        g = b0 - a1
+        if g > 0:
+            c, d = gaps.get((ka, kb), (0, 0))
+            gaps[(ka, kb)] = (c + 1, d + g)
+    if gaps:
+        total = sum(d for _, d in gaps.values()) / 1e6
+        out += ["", f"Idle gaps between dispatches in the timed sweeps: {total:.1f} ms in all; "
+                    f"the largest by the kernels before and after:", "",
+                "| before | after | gaps | total ms | avg us |", "|---|---|---|---|---|"]
+        for (ka, kb), (c, d) in sorted(gaps.items(), key=lambda kv: -kv[1][1])[:top]:
+            out.append(f"| `{ka}` | `{kb}` | {c} | {d / 1e6:.2f} | {d / c / 1e3:.1f} |")
+    return out
 
 
 PMC_COPY_ELEMS = 4096 * 4096   # complex128 elements of the calibration copy (256 MiB)

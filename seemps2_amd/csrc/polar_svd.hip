@@ -284,10 +284,11 @@ __device__ bool polar_barrier(PolarCtrl* c, unsigned int* phase, int* s_ok,
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator item r is
 // D[l / 16 + 4 r][l % 16].
 //
-// HERM_C: the product is Hermitian (A^H A with B = A): only the tiles on and
-// above the diagonal (tile row <= tile column) are computed.  HERM_B: B is
-// such a matrix, so its element (k, j) below the diagonal tiles is read as
-// conj(B[j][k]).
+// HERM_C: the product is Hermitian (A^H A with B = A, beta = 0): only the
+// tiles on and above the diagonal (tile row <= tile column) are computed, and
+// each is also stored, as its adjoint, below the diagonal.  HERM_B: B is a
+// matrix with only those upper tiles stored, so its element (k, j) below the
+// diagonal tiles is read as conj(B[j][k]).
 constexpr int PG_UNROLL = 8;
 constexpr int POLAR_THREADS = 512;
 constexpr int POLAR_WAVES = POLAR_THREADS / 64;
@@ -427,6 +428,9 @@ __device__ void polar_gemm(int64_t M64, int64_t N64, int64_t K64, const double2*
               v.y += beta * c0.y;
             }
             C[row * ldc + col] = v;
+            // HERM_C: the tile below the diagonal too, as the adjoint (the
+            // diagonal tiles hold both halves already)
+            if (HERM_C && m0 != n0) C[col * ldc + row] = make_double2(v.x, -v.y);
           }
         }
       }
@@ -680,7 +684,7 @@ __global__ __launch_bounds__(POLAR_THREADS) void polar_persistent(int64_t Lg, in
       const double g = (k == 0) ? f : 1.0;
       polar_gemm<true, true>(l, l, l, Z, l, Z, l, g * g, 0.0, nullptr, T, l, red);
       if (!(ok = polar_barrier(ctrl, &phase, &s_ok, trace))) break;
-      polar_gemm<false, false, true>(l, l, l, Z, l, T, l, -0.5 * g, 1.5 * g, Z, Z2, l, red);
+      polar_gemm<false>(l, l, l, Z, l, T, l, -0.5 * g, 1.5 * g, Z, Z2, l, red);
       if (!(ok = polar_barrier(ctrl, &phase, &s_ok, trace))) break;
       double2* tmp = Z;
       Z = Z2;
