@@ -15,6 +15,11 @@ Hamiltonian (tests/test_oracle_algorithms.py), as the reference pins
 ``to_mpo`` (tests/test_operators/test_hamiltonians.py:92-101).  MPO tensor
 entries themselves depend on the SVD gauge of degenerate singular values:
 parity unpinned for raw MPO entries (only the operator is compared).
+``random_mps`` is pinned by the reference's shape rules
+(tests/test_states/test_random_mps.py:14-27,64-75) and by its draw order
+against the generator's stream (tests/test_oracle_algorithms.py); the
+reference package cannot be imported here (its ``__init__`` imports h5py) to
+write golden tensors.
 """
 
 from __future__ import annotations

@@ -22,6 +22,31 @@ def test_random_mps_bond_rule():
     assert big.bond_dimensions() == [1] + [16] * 63 + [1]
 
 
+def test_random_mps_draw_order():
+    """factories.py:209-218 draws, site by site, the real part of each tensor
+    (shape (DL, d, DR), C order) and then its imaginary part from the one
+    generator.  So the oracle's tensors, read in that order, must be exactly the
+    generator's stream.  The reference package itself cannot be imported here
+    (seemps/__init__.py imports hdf5.py, which imports h5py, absent), so this
+    and the shape rules above are what pin the draw order."""
+    for dims, D, cplx in [([2] * 7, 5, True), ([3, 2, 4, 2], 6, True), ([2] * 5, 3, False)]:
+        psi = bl.random_mps(dims, D=D, complex=cplx, rng=np.random.default_rng(77))
+        parts = []
+        for A in psi.tensors:
+            parts.append(np.ascontiguousarray(A.real).ravel())
+            if cplx:
+                parts.append(np.ascontiguousarray(A.imag).ravel())
+        drawn = np.concatenate(parts)
+        stream = np.random.default_rng(77).normal(size=drawn.size)
+        assert np.array_equal(drawn, stream)
+        # a planted mistake (imaginary part drawn first) breaks it
+        if cplx:
+            swapped = []
+            for A in psi.tensors:
+                swapped += [A.imag.ravel(), A.real.ravel()]
+            assert not np.array_equal(np.concatenate(swapped), stream)
+
+
 @pytest.mark.parametrize("N", [2, 3, 6])
 def test_nn_mpo_matches_dense(N):
     for h in (bl.heisenberg_interaction(), bl.xxz_interaction(1.0, 0.5)):
