@@ -128,6 +128,24 @@ def test_dmrg_spin32_heisenberg_d4(S, rng):
     assert abs(out.energy - E0) < 1e-10 * abs(E0), (out.energy, E0)
 
 
+def test_dmrg_spin32_heisenberg_d4_learned_splits(S, rng):
+    """The same d = 4 chain (N = 8, bond up to 256) with the learned-rank
+    basis-sketched splits: the energy still matches exact diagonalisation."""
+    from seemps2_amd import learned
+
+    N = 8
+    h = spin32_heisenberg()
+    H = S.ConstantTIHamiltonian(N, interaction=h).to_mpo()
+    E0 = ex.ground_energy_sparse(ex.nn_hamiltonian_sparse([h] * (N - 1), [4] * N))
+    guess = ob.random_mps([4] * N, D=8, complex=True, rng=rng)
+    lt = learned.LearnedTruncation(learned.RankPredictor(seed=3), min_size=32)
+    with learned.learned_truncation(lt):
+        out = S.dmrg(H, S.MPS(guess.tensors), maxiter=14,
+                     strategy=S.Strategy(max_bond_dimension=256))
+    assert lt.calls > 0 and lt.calls > lt.full_splits  # sketched splits ran
+    assert abs(out.energy - E0) < 1e-10 * abs(E0), (out.energy, E0, lt)
+
+
 def test_truncate_long_spectrum(S, rng):
     """Spectra longer than the LDS-staged truncation (4096 values) use the
     unstaged kernel, with the same decisions as the oracle's loops."""

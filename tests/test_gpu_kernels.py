@@ -495,6 +495,29 @@ def test_svd_split_basis_grows_and_caps(K, rng, side):
     _check_split(K, An, Ad, Bd, D, mode, Ar, Br, 1e-12)
 
 
+@pytest.mark.parametrize("extra", [0, 16])
+def test_svd_split_basis_unrelated_basis(K, rng, extra):
+    """A basis unrelated to A's row space (random isometry): the sketched
+    matrix's columns are far from orthogonal, so the rotation iteration gives
+    up (or the block Jacobi finishes it), the truncation cannot certify the
+    sketch, and the full split decides: the result is the oracle's split."""
+    m, n, k = 300, 280, 120
+    sv = np.exp(-0.05 * np.arange(min(m, n)))
+    U, _ = np.linalg.qr(crandn(rng, m, m))
+    V, _ = np.linalg.qr(crandn(rng, n, n))
+    A = (U[:, : len(sv)] * sv) @ V[:, : len(sv)].conj().T
+    Q, _ = np.linalg.qr(crandn(rng, n, k))
+    basis = Q.conj().T.copy()  # k x n, orthonormal rows, unrelated to A
+    st = ocore.Strategy(tolerance=1e-10, max_bond_dimension=k)
+    Ar, Br, er = ocore.left_orth_2site(A.reshape(m, 1, 1, n), st)
+    Ad, Bd, D, err2, _, fb = K.svd_split_basis(K.as_dev(A), m, n, 0, K.as_dev(basis), k, extra,
+                                               0, st.method, st.tolerance,
+                                               st.max_bond_dimension, seed=3)
+    assert fb
+    _check_split(K, A, Ad, Bd, D, 0, Ar, Br, 1e-12)
+    assert abs(np.sqrt(err2) - er) < 1e-12 * np.linalg.norm(A)
+
+
 def test_svd_split_basis_too_wide_falls_back(K, rng):
     m, n, k = 64, 48, 40
     A, basis = _in_span(rng, m, n, k, 0)
