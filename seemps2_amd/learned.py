@@ -238,7 +238,12 @@ class LearnedTruncation:
             # back); one that grows more than predicted is caught by the
             # truncation (the weight outside the sketch exceeds the budget)
             # and split in full
-            p = min(max(self.oversample, l - k), MAX_EXPANSION)
+            # as many directions as the predicted growth of the bond (the
+            # unmargined prediction less the basis), at least the oversample:
+            # a bond that holds its size needs only the oversample, and every
+            # extra direction lengthens the sketched SVD and the expansion
+            # rotation
+            p = min(max(self.oversample, need - k), MAX_EXPANSION)
             if k + p < r:
                 return self._basis_split(AA2, m, n, mode, basis, k, p, strategy)
         if l >= r:
