@@ -79,7 +79,56 @@ def parse():
                          "average over the timed-region dispatches")
     ap.add_argument("--bench-json", metavar="FILE",
                     help="file holding this script's JSON line from the profiled run")
+    ap.add_argument("--gemm-shapes", action="store_true",
+                    help="time zgemm on the sweep's GEMM shapes (100 launches each) and "
+                         "exit; SEEMPS_GEMM_TILE / SEEMPS_GEMM_SCHEME select the variant")
     return ap.parse_args()
+
+
+# (label, M, N, K, opA, opB): the products of a chi=256 (and 128) sweep
+GEMM_SHAPES = [
+    ("matvec L.v, chi=256", 1280, 1024, 256, 0, 0),
+    ("matvec second product, chi=256 (split-K)", 1024, 256, 1280, 0, 0),
+    ("environment, op (C, N), chi=256", 256, 1280, 512, 3, 0),
+    ("environment, chi=256", 1280, 512, 256, 0, 0),
+    ("matvec L.v, chi=128", 640, 512, 128, 0, 0),
+    ("matvec second product, chi=128 (split-K)", 512, 128, 640, 0, 0),
+    ("sketch product", 512, 272, 512, 0, 0),
+    ("large square", 2048, 2048, 512, 0, 0),
+]
+
+
+def gemm_shapes(reps=100):
+    """Time K.gemm (the library's zgemm, split-K where it applies) on
+    GEMM_SHAPES: torch events around `reps` back-to-back launches on the
+    stream the library launches on.  One line per shape: time, TFLOP/s in
+    the 8 MNK count, and the relative difference to torch's product."""
+    import torch
+    from seemps2_amd import kernels as K
+
+    tile = os.environ.get("SEEMPS_GEMM_TILE", "auto")
+    scheme = "4m" if os.environ.get("SEEMPS_GEMM_SCHEME") == "4m" else "3m"  # 64x64: 4M
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    for label, M, N, Kd, oa, ob in GEMM_SHAPES:
+        A = torch.randn((Kd, M) if oa in (1, 2) else (M, Kd), dtype=torch.complex128,
+                        device="cuda", generator=gen)
+        B = torch.randn((N, Kd) if ob in (1, 2) else (Kd, N), dtype=torch.complex128,
+                        device="cuda", generator=gen)
+        C = K.gemm(A, B, oa, ob)
+        ref = (A.conj() if oa == 3 else A) @ B
+        err = ((C - ref).abs().max() / ref.abs().max()).item()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            K.gemm(A, B, oa, ob, C=C)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        tf = 8.0 * M * N * Kd / (us * 1e-6) / 1e12
+        print(f"[tile {tile}, 32x32 tile scheme {scheme}] {label}: {M}x{N}x{Kd} {us:.1f} us "
+              f"{tf:.2f} TFLOP/s "
+              f"rel. diff {err:.1e}", flush=True)
 
 
 def kernel_base_name(name):
@@ -876,6 +925,9 @@ def main():
         return
     if args.pmc_mfma:
         print(json.dumps(pmc_mfma(args.pmc_mfma), indent=1))
+        return
+    if args.gemm_shapes:
+        gemm_shapes()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the DMRG sweep is "

@@ -1,11 +1,20 @@
 // Complex128 GEMM on the FP64 matrix cores of gfx950 (v_mfma_f64_16x16x4f64).
 //
 // C = alpha * op(A) @ op(B) + beta * C for strided batches, row-major storage.
-// A complex product is four real MFMA products ("4M"): re += ar*br - ai*bi,
-// im += ar*bi + ai*br.  Operand tiles are staged in LDS split into real and
-// imaginary planes, K-major, so each MFMA fragment read is a contiguous run
-// of 16 doubles; the op(.) transposition / conjugation is folded into the
-// global->LDS load (no transposed copies are materialised).
+// A complex product is three real MFMA products ("3M", Gauss): t1 += ar*br,
+// t2 += ai*bi, t3 += (ar+ai)*(br+bi), then re = t1 - t2, im = t3 - t1 - t2;
+// or four ("4M"): re += ar*br - ai*bi, im += ar*bi + ai*br.  3M issues a
+// quarter fewer MFMAs for the same output; its rounding error is bounded, like
+// 4M's, by a small multiple of u * sum |a||b| (normwise), though the imaginary
+// part alone is not resolved to relative precision when it is much smaller
+// than the real part.  The FP64 32 x 32 tile uses 3M unless
+// SEEMPS_GEMM_SCHEME=4m; the 64 x 64 tile (no registers to spare for a third
+// accumulator) and FP32 use 4M.
+//
+// Operand tiles are staged in LDS split into real and imaginary planes,
+// K-major, so each MFMA fragment read is a contiguous run of 16 doubles; the
+// op(.) transposition / conjugation is folded into the global->LDS load (no
+// transposed copies are materialised).
 //
 // Tiles: 64 x 64 or 32 x 32 outputs per 256-thread workgroup (4 waves of a
 // quarter each), chosen per call to spread the tiles evenly over the CUs;
@@ -163,7 +172,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
 }
 
 // Workgroups per CU each tile shape is compiled for (bounds the registers).
-constexpr int tile_occupancy(int tm, int tn) { return tm == 64 && tn == 64 ? 3 : 6; }
+// The 3M 32 x 32 kernel holds a third accumulator and the operand sums: five
+// workgroups per CU keep it free of spills.
+constexpr int tile_occupancy(int tm, int tn, bool g3) {
+  return tm == 64 && tn == 64 ? 3 : (g3 ? 5 : 6);
+}
 
 // One workgroup of 4 waves per TM x TN output tile; the waves split it 2 x 2,
 // each owning (TM/2) x (TN/2) = FA x FB fragments of 16 x 16.  Batch index z
@@ -171,8 +184,8 @@ constexpr int tile_occupancy(int tm, int tn) { return tm == 64 && tn == 64 ? 3 :
 // final slice being shorter).  Tiles are numbered along blockIdx.x,
 // row-major over the ceil(M/TM) x ceil(N/TN) grid.
 // 64 x 64: at most 168 registers per lane (three waves per SIMD, no spills).
-template <int OPA, int OPB, typename P, int TM, int TN>
-__global__ __launch_bounds__(256, tile_occupancy(TM, TN)) void zgemm_kernel(
+template <int OPA, int OPB, typename P, int TM, int TN, bool G3>
+__global__ __launch_bounds__(256, tile_occupancy(TM, TN, G3)) void zgemm_kernel(
     int64_t M, int64_t N, int64_t Kfull, int64_t Klast, double2 alpha,
     const double2* __restrict__ A, int64_t lda, int64_t sA, const double2* __restrict__ B,
     int64_t ldb, int64_t sB, double2 beta, double2* __restrict__ C, int64_t ldc, int64_t sC) {
@@ -197,7 +210,9 @@ __global__ __launch_bounds__(256, tile_occupancy(TM, TN)) void zgemm_kernel(
   const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
   const int li = lane & 15, lk = lane >> 4;
 
-  V cr[FA][FB], ci[FA][FB];
+  // 4M: cr, ci hold re, im.  3M: cr, ci, cs hold t1, t2, t3.
+  constexpr int SA = G3 ? FA : 1, SB = G3 ? FB : 1;
+  V cr[FA][FB], ci[FA][FB], cs[SA][SB];
 #pragma unroll
   for (int a = 0; a < FA; ++a)
 #pragma unroll
@@ -205,6 +220,10 @@ __global__ __launch_bounds__(256, tile_occupancy(TM, TN)) void zgemm_kernel(
       cr[a][b] = V{0, 0, 0, 0};
       ci[a][b] = V{0, 0, 0, 0};
     }
+#pragma unroll
+  for (int a = 0; a < SA; ++a)
+#pragma unroll
+    for (int b = 0; b < SB; ++b) cs[a][b] = V{0, 0, 0, 0};
 
   // K-tiles of interior output tiles are loaded without bounds tests
   const bool interior = (m0 + TM <= M) && (n0 + TN <= N);
@@ -238,22 +257,34 @@ __global__ __launch_bounds__(256, tile_occupancy(TM, TN)) void zgemm_kernel(
         br[b] = bs_re[k * LB + wn + b * 16 + li];
         bi[b] = bs_im[k * LB + wn + b * 16 + li];
       }
-      // the real and imaginary chains alternate, so no product waits on the
-      // one issued just before it
+      if constexpr (G3) {
+        // three independent chains: no product waits on the one before it
 #pragma unroll
-      for (int a = 0; a < FA; ++a)
+        for (int a = 0; a < SA; ++a)
 #pragma unroll
-        for (int b = 0; b < FB; ++b) {
-          cr[a][b] = P::mfma(ar[a], br[b], cr[a][b]);
-          ci[a][b] = P::mfma(ar[a], bi[b], ci[a][b]);
-        }
+          for (int b = 0; b < SB; ++b) {
+            cr[a][b] = P::mfma(ar[a], br[b], cr[a][b]);
+            ci[a][b] = P::mfma(ai[a], bi[b], ci[a][b]);
+            cs[a][b] = P::mfma(ar[a] + ai[a], br[b] + bi[b], cs[a][b]);
+          }
+      } else {
+        // the real and imaginary chains alternate, so no product waits on the
+        // one issued just before it
 #pragma unroll
-      for (int a = 0; a < FA; ++a)
+        for (int a = 0; a < FA; ++a)
 #pragma unroll
-        for (int b = 0; b < FB; ++b) {
-          cr[a][b] = P::mfma(-ai[a], bi[b], cr[a][b]);
-          ci[a][b] = P::mfma(ai[a], br[b], ci[a][b]);
-        }
+          for (int b = 0; b < FB; ++b) {
+            cr[a][b] = P::mfma(ar[a], br[b], cr[a][b]);
+            ci[a][b] = P::mfma(ar[a], bi[b], ci[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < FA; ++a)
+#pragma unroll
+          for (int b = 0; b < FB; ++b) {
+            cr[a][b] = P::mfma(-ai[a], bi[b], cr[a][b]);
+            ci[a][b] = P::mfma(ai[a], br[b], ci[a][b]);
+          }
+      }
     }
     __syncthreads();
   }
@@ -268,7 +299,13 @@ __global__ __launch_bounds__(256, tile_occupancy(TM, TN)) void zgemm_kernel(
         const int64_t m = m0 + wm + a * 16 + P::row(lk, r);
         const int64_t n = n0 + wn + b * 16 + li;
         if (m < M && n < N) {
-          const double2 acc = make_double2((double)cr[a][b][r], (double)ci[a][b][r]);
+          double2 acc;
+          if constexpr (G3) {
+            const double t1 = cr[a][b][r], t2 = ci[a][b][r];
+            acc = make_double2(t1 - t2, ((double)cs[a][b][r] - t1) - t2);
+          } else {
+            acc = make_double2((double)cr[a][b][r], (double)ci[a][b][r]);
+          }
           double2 out = cmul(alpha, acc);
           if (use_beta) out = cfma(beta, C[m * ldc + n], out);
           C[m * ldc + n] = out;
@@ -298,6 +335,16 @@ int choose_tile(int64_t M, int64_t N, int64_t batch, int num_cu) {
   return tiles64 >= 4 * cu ? TILE_64x64 : TILE_32x32;
 }
 
+// FP64 products use the 3M scheme unless SEEMPS_GEMM_SCHEME=4m (a tuning and
+// ablation knob, read once).
+bool gemm_scheme_3m() {
+  static const bool g3 = [] {
+    const char* e = getenv("SEEMPS_GEMM_SCHEME");
+    return !(e && !strcmp(e, "4m"));
+  }();
+  return g3;
+}
+
 template <int OPA, int OPB>
 void launch(int prec, int tile, int64_t batch, hipStream_t s, int64_t M, int64_t N, int64_t K,
             int64_t Klast, double2 alpha, const double2* A, int64_t lda, int64_t sA,
@@ -307,14 +354,17 @@ void launch(int prec, int tile, int64_t batch, hipStream_t s, int64_t M, int64_t
     return dim3((unsigned)(ceil_div(M, tm) * ceil_div(N, tn)), 1u, (unsigned)batch);
   };
   if (prec == SEEMPS_PREC_FP32)
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF32, 64, 64>), grid(64, 64), dim3(256), 0, s,
-                       M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF32, 64, 64, false>), grid(64, 64), dim3(256),
+                       0, s, M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+  else if (gemm_scheme_3m() && tile == TILE_32x32)
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64, 32, 32, true>), grid(32, 32), dim3(256),
+                       0, s, M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
   else if (tile == TILE_32x32)
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64, 32, 32>), grid(32, 32), dim3(256), 0, s,
-                       M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64, 32, 32, false>), grid(32, 32), dim3(256),
+                       0, s, M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
   else
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64, 64, 64>), grid(64, 64), dim3(256), 0, s,
-                       M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, PrecF64, 64, 64, false>), grid(64, 64), dim3(256),
+                       0, s, M, N, K, Klast, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
 }
 
 // C[m, n] = alpha * sum_z P[z][m, n] + beta * C[m, n]  (P packed M x N slices).

@@ -57,6 +57,76 @@ def test_zgemm_large_tile_path(K, rng, opA, opB):
     assert rel(K.to_numpy(out), ref) < 1e-14
 
 
+def _int_operands(rng, opA, opB, M, N, K_):
+    def ints(*shape):
+        return (rng.integers(-3, 4, size=shape) + 1j * rng.integers(-3, 4, size=shape)).astype(
+            np.complex128)
+
+    A = ints(*((K_, M) if opA in (1, 2) else (M, K_)))
+    B = ints(*((N, K_) if opB in (1, 2) else (K_, N)))
+    return A, B
+
+
+@pytest.mark.parametrize("opA", [0, 1, 2, 3])
+@pytest.mark.parametrize("opB", [0, 1, 2, 3])
+@pytest.mark.parametrize("M,N,K_", [(37, 53, 29), (130, 70, 200), (2065, 2088, 24)])
+def test_zgemm_exact_on_small_integers(K, rng, opA, opB, M, N, K_):
+    """Gaussian-integer operands: every partial sum of either product scheme
+    (3M on the 32 x 32 tile, 4M on the 64 x 64 one, the last shape) is an
+    exact integer, so the result must equal the exact product bit for bit.
+    A wrong sign or a dropped term of the 3M recombination fails here."""
+    A, B = _int_operands(rng, opA, opB, M, N, K_)
+    ref = OPS[opA](A) @ OPS[opB](B)
+    out = K.to_numpy(K.gemm(K.as_dev(A), K.as_dev(B), opA, opB))
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [(0, 0), (2, 1), (3, 3)])
+def test_zgemm_real_operands_give_real_products(K, rng, opA, opB):
+    """Real operands give an imaginary part of exactly zero: the 3M sum
+    product (ar + 0)(br + 0) repeats the ar br product operation for
+    operation, so t3 - t1 - t2 cancels exactly."""
+    M, N, K_ = 300, 200, 257
+    A = rng.normal(size=(K_, M) if opA in (1, 2) else (M, K_)).astype(np.complex128)
+    B = rng.normal(size=(N, K_) if opB in (1, 2) else (K_, N)).astype(np.complex128)
+    out = K.to_numpy(K.gemm(K.as_dev(A), K.as_dev(B), opA, opB))
+    assert np.all(out.imag == 0.0)
+    assert rel(out, OPS[opA](A) @ OPS[opB](B)) < 1e-14
+
+
+def test_zgemm_4m_scheme_knob(tmp_path):
+    """SEEMPS_GEMM_SCHEME=4m (read once per process) switches the 32 x 32
+    tile back to four real products: same results, in a process of its own."""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    script = tmp_path / "gemm4m.py"
+    script.write_text(
+        "import numpy as np\n"
+        "from seemps2_amd import kernels as K\n"
+        "rng = np.random.default_rng(5)\n"
+        "OPS = {0: lambda x: x, 1: lambda x: x.T, 2: lambda x: x.conj().T, 3: lambda x: x.conj()}\n"
+        "for opA, opB in [(0, 0), (2, 0), (1, 3), (3, 2)]:\n"
+        "    M, N, Kd = 130, 70, 200\n"
+        "    A = rng.integers(-3, 4, size=(Kd, M) if opA in (1, 2) else (M, Kd)) + 1j * "
+        "rng.integers(-3, 4, size=(Kd, M) if opA in (1, 2) else (M, Kd))\n"
+        "    B = rng.integers(-3, 4, size=(N, Kd) if opB in (1, 2) else (Kd, N)) + 1j * "
+        "rng.integers(-3, 4, size=(N, Kd) if opB in (1, 2) else (Kd, N))\n"
+        "    out = K.to_numpy(K.gemm(K.as_dev(A), K.as_dev(B), opA, opB))\n"
+        "    assert np.array_equal(out, OPS[opA](A) @ OPS[opB](B)), (opA, opB)\n"
+        "print('4m ok')\n"
+    )
+    root = Path(__file__).resolve().parents[1]
+    env = dict(os.environ, SEEMPS_GEMM_SCHEME="4m", SEEMPS_GEMM_TILE="32x32",
+               PYTHONPATH=str(root))
+    r = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "4m ok" in r.stdout
+
+
 @pytest.mark.parametrize("opA,opB", [(0, 0), (2, 0), (0, 1), (3, 2), (1, 3)])
 @pytest.mark.parametrize("M,N,K_", [(64, 48, 2048), (512, 512, 1280), (5, 1, 9000)])
 def test_zgemm_splitk_through_python(K, rng, opA, opB, M, N, K_):
