@@ -79,6 +79,9 @@ def parse():
                          "average over the timed-region dispatches")
     ap.add_argument("--bench-json", metavar="FILE",
                     help="file holding this script's JSON line from the profiled run")
+    ap.add_argument("--many-state-workload", type=int, metavar="COUNT",
+                    help="simplify the first COUNT sums of the many-state workload in the "
+                         "plain form and exit (the program of the many-state kernel trace)")
     ap.add_argument("--gemm-shapes", action="store_true",
                     help="time zgemm on the sweep's GEMM shapes (100 launches each) and "
                          "exit; SEEMPS_GEMM_TILE / SEEMPS_GEMM_SCHEME select the variant")
@@ -720,6 +723,46 @@ def batch_form(pairs=512, seed=7):
                 grouped_launches=bo.launches(), max_rel_diff=float(f"{rel:.2e}"))
 
 
+def many_state_workload(count, seed=9):
+    """The first `count` sums of many_simplify_form's workload, simplified in
+    the plain form one after the other (the program profiled for the
+    many-state kernel trace); prints the wall time of the simplify calls."""
+    import numpy as np
+    import torch
+    import seemps2_amd as S
+
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+
+    def state(N, chi, dtype, g=gen):
+        dims = [min(chi, 2 ** min(i, N - i)) for i in range(N + 1)]
+        return S.MPS([(torch.randn(dims[i], 2, dims[i + 1], dtype=torch.complex128,
+                                   device="cuda", generator=g) / np.sqrt(2 * dims[i]))
+                      .to(dtype).to(torch.complex128) for i in range(N)])
+
+    specs = []
+    for k in range(count):
+        N = int(rng.integers(10, 41))
+        specs.append((N, int(rng.integers(8, 129)), int(rng.integers(8, 129)),
+                      torch.complex64 if k % 2 else torch.complex128))
+    st = S.Strategy(tolerance=1e-8, simplification_tolerance=1e-8)
+    wgen = torch.Generator(device="cuda").manual_seed(seed + 1)
+    warm = [state(12, 16, torch.complex128, wgen) for _ in range(2)]
+    S.simplify(S.MPSSum([1.0, 1.0], warm), strategy=st)  # workspace growth, not timed
+    t_all = 0.0
+    for N, ca, cb, dt in specs:
+        a, b = state(N, ca, dt), state(N, cb, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = S.simplify(S.MPSSum([1.0, 1.0], [a, b]), strategy=st)
+        torch.cuda.synchronize()
+        t_all += time.perf_counter() - t0
+        log(f"many-state workload: N={N}, chi {ca} + {cb}: max bond "
+            f"{max(out.bond_dimensions())}")
+    log(f"many-state workload: {count} sums simplified in {t_all:.3f} s "
+        f"({count / t_all:.2f} states/s)")
+
+
 def many_simplify_form(net, pairs=512, seconds=20.0, seed=9):
     """Simplify of many uneven sums psi_k + phi_k (N ~ U[10, 40] sites, chi ~
     U[8, 128], half of them complex64) to tolerance 1e-8, state by state, in
@@ -928,6 +971,9 @@ def main():
         return
     if args.gemm_shapes:
         gemm_shapes()
+        return
+    if args.many_state_workload:
+        many_state_workload(args.many_state_workload)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the DMRG sweep is "
