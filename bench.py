@@ -584,8 +584,12 @@ def learned_form(QF, strategy, direction, steps, chi, spectra, value_plain, E_pl
 def dmrg_dropin(state, H, strategy, sweeps=2, lt=None, precision="fp64"):
     """The public dmrg() from a converged state, `sweeps` sweeps: the driver
     as a user calls it, with its environment set-up and the energy / variance
-    pass after every sweep (the unsimplified H psi at bond w chi).  Returns
-    the wall time per sweep and the energy."""
+    pass after every sweep (the unsimplified H psi at bond w chi).  tol = 0
+    and tol_up = 1 keep the stopping rule from ending it before `sweeps`;
+    tol_eigs = 1e-10 (dmrg's default tol, which tol_eigs follows when not
+    given) keeps the local eigensolves at the tolerance of the timed sweeps
+    instead of following tol = 0 down to machine precision.  Returns the wall
+    time per sweep and the energy."""
     import torch
     import seemps2_amd as S
     from seemps2_amd import kernels as K, learned
@@ -596,11 +600,11 @@ def dmrg_dropin(state, H, strategy, sweeps=2, lt=None, precision="fp64"):
     t0 = time.perf_counter()
     if lt is not None:
         with learned.learned_truncation(lt):
-            res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, strategy=strategy,
-                         precision=precision)
+            res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, tol_eigs=1e-10,
+                         strategy=strategy, precision=precision)
     else:
-        res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, strategy=strategy,
-                     precision=precision)
+        res = S.dmrg(H, guess, maxiter=sweeps, tol=0.0, tol_up=1.0, tol_eigs=1e-10,
+                     strategy=strategy, precision=precision)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     done = max(len(res.trajectory) - 1, 1)
@@ -1067,8 +1071,9 @@ def main():
         lt = _learned.LearnedTruncation(net, oversample=16, margin=1.25,
                                         power_iters=args.sketch_power_iters, min_size=64)
         d_chosen = dmrg_dropin(QF.state, H, st_d, lt=lt, precision=args.eig_precision)
-        dropin = dict(workload=f"seemps2_amd.dmrg(H, guess, maxiter=2) from the chi={chi} state, "
-                               "energy and variance of the unsimplified H psi after each sweep",
+        dropin = dict(workload=f"seemps2_amd.dmrg(H, guess, maxiter=2, tol=0, tol_up=1, "
+                               f"tol_eigs=1e-10) from the chi={chi} state, energy and variance "
+                               "of the unsimplified H psi after each sweep",
                       plain=d_plain, chosen=d_chosen,
                       speedup_vs_plain=round(d_plain["ms_per_sweep"] / d_chosen["ms_per_sweep"], 3),
                       energy_rel_diff=float(f"{abs(d_chosen['energy'] - d_plain['energy']) / abs(d_plain['energy']):.3e}"))
