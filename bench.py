@@ -59,6 +59,11 @@ def parse():
                          "(the program profiled by the rocprofv3 --pmc passes)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="reduce the two --pmc passes of --pmc-workload (no GPU work)")
+    ap.add_argument("--pmc-small-workload", action="store_true",
+                    help="run the small-chi MPO.apply + simplify between markers and a "
+                         "calibration copy, exit (the program of the small-chi --pmc passes)")
+    ap.add_argument("--pmc-small-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="reduce the two --pmc passes of --pmc-small-workload (no GPU work)")
     ap.add_argument("--pmc-mfma", metavar="DIR",
                     help="reduce an MFMA-counter --pmc pass of --pmc-workload (no GPU work)")
     ap.add_argument("--no-batch", action="store_true",
@@ -303,6 +308,82 @@ def pmc_workload(QF, H, strategy, chi, direction):
     torch.cuda.synchronize()
     log(f"pmc workload: bond updates at sites {sites} (chosen, then plain), "
         f"{lt.basis_splits} basis-sketched split(s)")
+
+
+def pmc_small_workload(seed=1001):
+    """Dispatches for the small-chi --pmc passes: a marker, the MPO.apply +
+    simplify of a random complex MPS (N = 20, d = 2, chi = 32, normalised;
+    Heisenberg MPO of bond 5; tolerance 1e-10, max bond 64, 2 sweeps), a
+    marker, then the calibration copy of pmc_workload."""
+    import numpy as np
+    import torch
+    import seemps2_amd as S
+    from seemps2_amd import _lib, kernels as K
+
+    def marker():
+        _lib.check(_lib.lib.seemps_timing_calibrate(1, _lib.stream()))
+
+    rng = np.random.default_rng(seed)
+    N = 20
+    psi = S.random_mps([2] * N, D=32, truncate=False, complex=True, rng=rng)
+    psi = S.MPS([K.scale(1.0 / np.sqrt(psi.norm_squared()), psi[0])] + list(psi)[1:])
+    W = S.HeisenbergHamiltonian(N).to_mpo()
+    st = S.Strategy(tolerance=1e-10, simplification_tolerance=1e-10, max_bond_dimension=64,
+                    max_sweeps=2)
+    W.apply(psi, strategy=st, simplify=True)  # workspace growth, outside the markers
+    torch.cuda.synchronize()
+    marker()
+    out = W.apply(psi, strategy=st, simplify=True)
+    marker()
+    x = K.empty(PMC_COPY_ELEMS)
+    y = K.empty(PMC_COPY_ELEMS)
+    K.fill_zero(x)
+    K.scale2d(x.view(4096, 4096), 4096, 4096, 4096, out=y.view(4096, 4096))
+    torch.cuda.synchronize()
+    log(f"pmc small workload: MPO.apply + simplify, N={N}, chi 32 -> bonds "
+        f"{max(out.bond_dimensions())}")
+
+
+def pmc_small_summary(fetch_dir, write_dir, top=12):
+    """Per kernel of --pmc-small-workload (between its two markers): dispatches,
+    HBM bytes per dispatch (each counter scaled by what it reads for the
+    calibration copy), average duration under the counters, and the achieved
+    rate against the 8 TB/s HBM peak; the same for all its kernels together."""
+    f, w = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    copy_kib = PMC_COPY_ELEMS * 16 / 1024
+    cf = copy_kib / [c["FETCH_SIZE"] for (_, k), c, _ in f if k == "zscale2d_kernel"][-1]
+    cw = copy_kib / [c["WRITE_SIZE"] for (_, k), c, _ in w if k == "zscale2d_kernel"][-1]
+
+    def between(rows):
+        marks = [i for i, ((_, k), _, _) in enumerate(rows) if k == PMC_MARKER][-2:]
+        if len(marks) != 2:
+            raise SystemExit(f"expected 2 {PMC_MARKER} markers, found {len(marks)}")
+        return rows[marks[0] + 1:marks[1]]
+
+    ff, ww = between(f), between(w)
+    if [k for (_, k), _, _ in ff] != [k for (_, k), _, _ in ww]:
+        raise SystemExit("the two passes dispatched different kernels")
+    kern = {}
+    for ((_, k), cfv, d), (_, cwv, _) in zip(ff, ww):
+        r = kern.setdefault(k, {"dispatches": 0, "bytes": 0.0, "ns": 0})
+        r["dispatches"] += 1
+        r["bytes"] += (cfv["FETCH_SIZE"] * cf + cwv["WRITE_SIZE"] * cw) * 1024
+        r["ns"] += d
+    total_b = sum(r["bytes"] for r in kern.values())
+    total_ns = sum(r["ns"] for r in kern.values())
+    rows = []
+    for k, r in sorted(kern.items(), key=lambda kv: -kv[1]["ns"])[:top]:
+        gbs = r["bytes"] / r["ns"] if r["ns"] else 0.0
+        rows.append({"kernel": k, "dispatches": r["dispatches"],
+                     "bytes_per_dispatch": round(r["bytes"] / r["dispatches"]),
+                     "us_per_dispatch": round(r["ns"] / r["dispatches"] / 1e3, 2),
+                     "gb_per_s": round(gbs, 1), "frac_of_8tb": round(gbs / 8000.0, 4)})
+    return {"fetch_scale": round(cf, 4), "write_scale": round(cw, 4),
+            "all_kernels": {"dispatches": sum(r["dispatches"] for r in kern.values()),
+                            "bytes": round(total_b), "kernel_ns": total_ns,
+                            "gb_per_s": round(total_b / total_ns, 1) if total_ns else 0.0,
+                            "frac_of_8tb": round(total_b / total_ns / 8000.0, 4) if total_ns else 0.0},
+            "kernels": rows}
 
 
 PMC_MARKER = "timing_empty_kernel"
@@ -970,6 +1051,9 @@ def main():
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
         return
+    if args.pmc_small_summary:
+        print(json.dumps(pmc_small_summary(*args.pmc_small_summary), indent=1))
+        return
     if args.pmc_mfma:
         print(json.dumps(pmc_mfma(args.pmc_mfma), indent=1))
         return
@@ -978,6 +1062,9 @@ def main():
         return
     if args.many_state_workload:
         many_state_workload(args.many_state_workload)
+        return
+    if args.pmc_small_workload:
+        pmc_small_workload()
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the DMRG sweep is "
