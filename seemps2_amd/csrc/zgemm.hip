@@ -315,11 +315,12 @@ __global__ __launch_bounds__(256, tile_occupancy(TM, TN, G3)) void zgemm_kernel(
 
 // Tile shape for an M x N output in `batch` slices on `num_cu` CUs.  The
 // 64 x 64 tile (three workgroups per CU) is the faster one per output element
-// once every CU holds four or more of them; below that the 32 x 32 tile (six
-// per CU, four times as many tiles) spreads the work more evenly and keeps
-// more waves per SIMD to hide the LDS and memory latencies.  Measured on the
-// matvec and environment GEMMs of chi = 128 and 256 (25-35 % faster with the
-// small tile) and on a 2048 x 2048 x 512 product (6 % slower with it).
+// once every CU holds four or more of them; below that the 32 x 32 tile (five
+// per CU with the three-product scheme, six with four; four times as many
+// tiles) spreads the work more evenly and keeps more waves per SIMD to hide
+// the LDS and memory latencies.  Measured with four products on the matvec
+// and environment GEMMs of chi = 128 and 256 (25-35 % faster with the small
+// tile) and on a 2048 x 2048 x 512 product (6 % slower with it).
 // SEEMPS_GEMM_TILE = 64x64 or 32x32 fixes the shape (a tuning knob).
 int choose_tile(int64_t M, int64_t N, int64_t batch, int num_cu) {
   static const int forced = [] {
